@@ -1,0 +1,547 @@
+// api.cpp -- the C ABI (include/cumodp.h, include/cumodp_amd.h) and the
+// C++-linkage drop-ins (include/stockham.h, list_stockham.h, fft_aux.h,
+// list_inv_stockham.h) over the gfx950 kernels.
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cumodp.h"
+#include "../../include/cumodp_amd.h"
+#include "../../include/fft_aux.h"
+#include "../../include/list_inv_stockham.h"
+#include "../../include/list_stockham.h"
+#include "../../include/stockham.h"
+#include "internal.h"
+
+using namespace cm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+cumodp_err fail(cumodp_err code, const char *what, hipError_t e = hipSuccess) {
+    g_last_error = what;
+    if (e != hipSuccess) {
+        g_last_error += ": ";
+        g_last_error += hipGetErrorString(e);
+    }
+    return code;
+}
+
+int ceil_log2(int64_t s) {
+    int e = 0;
+    while (((int64_t)1 << e) < s) ++e;
+    return e;
+}
+
+hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+// Checks shared by every transform entry point.
+cumodp_err check_prime_size(u32 p, int k) {
+    if (p < 3 || (p & 1) == 0 || p >= (1u << 31)) return fail(CUMODP_ASSUMPTION_ERROR, "modulus must be an odd prime below 2^31");
+    if (k > kMaxLog) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "transform size above 2^26");
+    if (k > cm_fourier_degree(p)) return fail(CUMODP_FOURIER_DEGREE_TOO_SMALL, "2^k does not divide p-1");
+    return CUMODP_SUCCESS;
+}
+
+// ---- univariate multiply --------------------------------------------------
+// Hd[0 .. nout) = Fd * Gd mod p; Fd, Gd hold lf, lg words (device).
+cumodp_err mul_dev_impl(u32 *Hd, int64_t nout, const u32 *Fd, int64_t lf, const u32 *Gd,
+                        int64_t lg, u32 p, hipStream_t s) {
+    const int64_t slen = lf + lg - 1;
+    int e = ceil_log2(slen);
+    if (e < 4) e = 4;   // smallest workgroup transform; zero padding is harmless
+    cumodp_err ce = check_prime_size(p, e);
+    if (ce != CUMODP_SUCCESS) return ce;
+    const Family *fam = family_default(p);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    const int64_t n = (int64_t)1 << e;
+    hipError_t he;
+    if (e <= kMaxLdsLog) {
+        MulDesc d;
+        d.f = 0; d.g = 0; d.h = 0;
+        d.lf = (int)lf; d.lg = (int)lg; d.lh = (int)(nout < n ? nout : n); d.logn = e;
+        he = launch_mul_lds_one(e, Fd, Gd, Hd, nullptr, 1, d, fam, s);
+    } else {
+        u32 *wf = (u32 *)workspace(0, sizeof(u32) * (size_t)n);
+        u32 *wg = (u32 *)workspace(1, sizeof(u32) * (size_t)n);
+        if (!wf || !wg) return fail(CUMODP_OUT_OF_MEMORY, "multiply workspace");
+        he = hipMemcpyAsync(wf, Fd, sizeof(u32) * (size_t)lf, hipMemcpyDeviceToDevice, s);
+        if (he == hipSuccess) he = launch_fill(wf + lf, n - lf, 0, s);
+        if (he == hipSuccess) he = hipMemcpyAsync(wg, Gd, sizeof(u32) * (size_t)lg, hipMemcpyDeviceToDevice, s);
+        if (he == hipSuccess) he = launch_fill(wg + lg, n - lg, 0, s);
+        if (he == hipSuccess) he = launch_mul_four_step(e, wf, wg, 1, fam, s);
+        const int64_t nw = nout < n ? nout : n;
+        if (he == hipSuccess) he = hipMemcpyAsync(Hd, wf, sizeof(u32) * (size_t)nw, hipMemcpyDeviceToDevice, s);
+    }
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "multiply", he);
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err mul_host_impl(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg, const sfixn *G,
+                         sfixn p) {
+    if (df < 0 || dg < 0 || dh < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative degree");
+    const int64_t s = (int64_t)df + dg + 1;
+    const int e = ceil_log2(s);
+    const int64_t n = (int64_t)1 << e;
+    if (n < 256) return fail(CUMODP_FFT_SIZE_TOO_SMALL, "fft size < 256 is not supported by gpu");
+    cumodp_err ce = check_prime_size((u32)p, e);
+    if (ce != CUMODP_SUCCESS) return ce;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const int64_t nout = (int64_t)dh + 1 < n ? (int64_t)dh + 1 : n;
+    u32 *buf = (u32 *)workspace(2, sizeof(u32) * (size_t)(df + 1 + dg + 1 + nout));
+    if (!buf) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    u32 *dF = buf, *dG = buf + df + 1, *dH = dG + dg + 1;
+    hipStream_t st = 0;
+    hipError_t he = hipMemcpyAsync(dF, F, sizeof(u32) * (size_t)(df + 1), hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dG, G, sizeof(u32) * (size_t)(dg + 1), hipMemcpyHostToDevice, st);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    ce = mul_dev_impl(dH, nout, dF, df + 1, dG, dg + 1, (u32)p, st);
+    if (ce != CUMODP_SUCCESS) return ce;
+    he = hipMemcpyAsync(H, dH, sizeof(u32) * (size_t)nout, hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    return CUMODP_SUCCESS;
+}
+
+// ---- transforms ------------------------------------------------------------
+// m rows of 2^k words in place: natural-order DFT with root w (inverse: root
+// w^-1 and scaling by 2^-k).
+cumodp_err ntt_dev_impl(u32 *X, int64_t m, int k, u32 w, u32 p, bool inverse, hipStream_t s) {
+    if (k < 4) return fail(CUMODP_FFT_SIZE_TOO_SMALL, "transform size below 16");
+    cumodp_err ce = check_prime_size(p, k);
+    if (ce != CUMODP_SUCCESS) return ce;
+    const Family *fam = family_for_root(p, w % p, k);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    hipError_t he;
+    if (k <= kMaxLdsLog) {
+        uint2 sc = make_uint2(1u, 0u);
+        if (inverse) {
+            u32 ni = cm_host_inv(cm_host_pow(2, k, p), p);
+            sc = make_uint2(ni, cm_host_shoup_q(ni, p));
+        }
+        he = launch_ntt_lds(k, X, m, inverse ? fam->inv : fam->fwd, sc, fam, s);
+    } else {
+        const int64_t n = (int64_t)1 << k;
+        if (m > INT_MAX) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "too many rows");
+        u32 *Y = (u32 *)workspace(3, sizeof(u32) * (size_t)(n * m));
+        if (!Y) return fail(CUMODP_OUT_OF_MEMORY, "transform workspace");
+        he = launch_ntt_four_step(k, X, Y, (int)m, inverse, fam, s);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "transform", he);
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err ntt_host_impl(sfixn *X, int64_t m, int k, u32 w, u32 p, bool inverse) {
+    if (k < 4) return fail(CUMODP_FFT_SIZE_TOO_SMALL, "transform size below 16");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const size_t bytes = sizeof(u32) * (size_t)(m << k);
+    u32 *d = (u32 *)workspace(4, bytes);
+    if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    hipError_t he = hipMemcpy(d, X, bytes, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    cumodp_err ce = ntt_dev_impl(d, m, k, w, p, inverse, 0);
+    if (ce != CUMODP_SUCCESS) return ce;
+    he = hipMemcpy(X, d, bytes, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    return CUMODP_SUCCESS;
+}
+
+// DFT_{m,n} on a row-major 2^em x 2^en device array: rows with root wn,
+// then columns with root wm (src/list_stockham.cu:1161-1204); the inverse
+// uses the inverse roots and scales by (mn)^-1.
+cumodp_err biv_dev_impl(u32 *X, int em, u32 wm, int en, u32 wn, u32 p, bool inverse,
+                        hipStream_t s) {
+    if (em < 1 || em > kMaxLdsLog) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "2^em must lie in [2, 2^13]");
+    cumodp_err ce = check_prime_size(p, em);
+    if (ce != CUMODP_SUCCESS) return ce;
+    if (inverse) { wm = cm_host_inv(wm % p, p); wn = cm_host_inv(wn % p, p); }
+    ce = ntt_dev_impl(X, (int64_t)1 << em, en, wn, p, false, s);
+    if (ce != CUMODP_SUCCESS) return ce;
+    const Family *fm = family_for_root(p, wm % p, em);
+    if (!fm) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    uint2 sc = make_uint2(1u, 0u);
+    if (inverse) {
+        u32 ni = cm_host_inv(cm_host_pow(2, em + en, p), p);
+        sc = make_uint2(ni, cm_host_shoup_q(ni, p));
+    }
+    hipError_t he = launch_col_ntt(em, X, (int64_t)1 << en, fm->fwd, sc, fm, s);
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "bivariate transform", he);
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err biv_host_impl(sfixn *X, int em, u32 wm, int en, u32 wn, u32 p, bool inverse) {
+    if (em < 1 || en < 4 || em + en > 30) return fail(CUMODP_ASSUMPTION_ERROR, "bivariate sizes");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const size_t bytes = sizeof(u32) << (em + en);
+    u32 *d = (u32 *)workspace(4, bytes);
+    if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    hipError_t he = hipMemcpy(d, X, bytes, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    cumodp_err ce = biv_dev_impl(d, em, wm, en, wn, p, inverse, 0);
+    if (ce != CUMODP_SUCCESS) return ce;
+    he = hipMemcpy(X, d, bytes, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    return CUMODP_SUCCESS;
+}
+
+// root of order 2^k of the library's default family
+u32 default_root(u32 p, int k) {
+    const Family *f = family_default(p);
+    if (!f || k > f->K) return 0;
+    return cm_host_pow(f->wN, (u64)1 << (f->K - k), p);
+}
+
+u32 read_root(const sfixn *W) {
+    sfixn w = 1;
+    (void)hipMemcpy(&w, W + 1, sizeof(w), hipMemcpyDeviceToHost);
+    return (u32)w;
+}
+
+void report(const char *fn, cumodp_err e) {
+    if (e != CUMODP_SUCCESS) fprintf(stderr, "%s: %s\n", fn, g_last_error.c_str());
+}
+
+}  // namespace
+
+// ===========================================================================
+// C linkage: include/cumodp.h
+// ===========================================================================
+extern "C" {
+
+int is_cuda_enabled(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c > 0;
+}
+
+int num_of_cuda_devices(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int is_double_float_enabled(void) { return is_cuda_enabled(); }
+
+static size_t total_mem() {
+    size_t fr = 0, tot = 0;
+    if (!is_cuda_enabled() || hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return tot;
+}
+
+unsigned int global_memory_in_bytes(void) {
+    size_t t = total_mem();
+    return t > UINT_MAX ? UINT_MAX : (unsigned int)t;
+}
+
+float global_memory_in_megabytes(void) { return (float)(total_mem() >> 20); }
+
+int can_call_fftmul_uni(sfixn df, sfixn dg) {
+    size_t mem = total_mem();
+    if (!mem || df < 0 || dg < 0) return 0;
+    int b = ceil_log2((int64_t)df + dg + 1);
+    if (b > kMaxLog) return 0;
+    int lm = 0;
+    while (((size_t)2 << lm) <= mem / 4) ++lm;   // floor(log2(mem/4))
+    return lm > b + 2;
+}
+
+int can_call_subres2(sfixn, sfixn, sfixn) { return 0; }
+int can_call_subres3(sfixn, sfixn, sfixn, sfixn) { return 0; }
+
+cumodp_err cumodp_fft_uni(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
+    if (n != ((sfixn)1 << k)) return fail(CUMODP_ASSUMPTION_ERROR, "n != 2^k");
+    return ntt_host_impl(X, 1, k, (u32)w, (u32)p, false);
+}
+
+cumodp_err cumodp_invfft_uni(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
+    if (n != ((sfixn)1 << k)) return fail(CUMODP_ASSUMPTION_ERROR, "n != 2^k");
+    return ntt_host_impl(X, 1, k, (u32)w, (u32)p, true);
+}
+
+cumodp_err cumodp_fftmul_uni(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg,
+                             const sfixn *G, sfixn p) {
+    return mul_host_impl(dh, H, df, F, dg, G, p);
+}
+
+cumodp_err cumodp_fft_bivariate(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, sfixn p) {
+    return biv_host_impl(X, em, (u32)wm, en, (u32)wn, (u32)p, false);
+}
+
+cumodp_err cumodp_invfft_bivariate(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, sfixn p) {
+    return biv_host_impl(X, em, (u32)wm, en, (u32)wn, (u32)p, true);
+}
+
+sfixn cumodp_subres_chain2_fine(sfixn *, sfixn, sfixn, sfixn, sfixn, const sfixn *, sfixn, sfixn,
+                                const sfixn *, sfixn) { return -1; }
+sfixn cumodp_subres_chain2_coarse(sfixn *, sfixn, sfixn, sfixn, sfixn, const sfixn *, sfixn,
+                                  sfixn, const sfixn *, sfixn) { return -1; }
+sfixn cumodp_subres_chain3_fine(sfixn *, sfixn, sfixn, sfixn, sfixn, sfixn, sfixn, sfixn,
+                                const sfixn *, sfixn, sfixn, sfixn, const sfixn *, sfixn) { return -1; }
+sfixn cumodp_subres_chain3_coarse(sfixn *, sfixn, sfixn, sfixn, sfixn, sfixn, sfixn, sfixn,
+                                  const sfixn *, sfixn, sfixn, sfixn, const sfixn *, sfixn) { return -1; }
+void *init_cuda_scube(sfixn, const sfixn *, const sfixn *, sfixn) { return nullptr; }
+void free_cuda_scube(void *) {}
+void print_cuda_scube(const void *) {}
+cumodp_err build_cuda_scube(void *, const sfixn *, const sfixn *, const sfixn *, const sfixn *) {
+    return CUMODP_FAILURE;
+}
+const sfixn *interp_subres_coeff2(sfixn *, void *, sfixn, sfixn) { return nullptr; }
+const sfixn *interp_subres_coeff3(sfixn *, sfixn *, void *, sfixn, sfixn) { return nullptr; }
+
+// ===========================================================================
+// C linkage: include/cumodp_amd.h
+// ===========================================================================
+void *cumodp_dev_alloc(size_t bytes) {
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, bytes ? bytes : 4);
+    if (e != hipSuccess) { fail(CUMODP_OUT_OF_MEMORY, "hipMalloc", e); return nullptr; }
+    return d;
+}
+
+cumodp_err cumodp_dev_free(void *ptr) {
+    hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "hipFree", e);
+}
+
+cumodp_err cumodp_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "memcpy h2d", e);
+}
+
+cumodp_err cumodp_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "memcpy d2h", e);
+}
+
+cumodp_err cumodp_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "memcpy d2d", e);
+}
+
+cumodp_err cumodp_synchronize(void) {
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "synchronize", e);
+}
+
+void *cumodp_stream_create(void) {
+    hipStream_t s = nullptr;
+    if (hipStreamCreate(&s) != hipSuccess) return nullptr;
+    return (void *)s;
+}
+
+cumodp_err cumodp_stream_destroy(void *stream) {
+    hipError_t e = hipStreamDestroy(as_stream(stream));
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "stream destroy", e);
+}
+
+cumodp_err cumodp_stream_synchronize(void *stream) {
+    hipError_t e = hipStreamSynchronize(as_stream(stream));
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "stream synchronize", e);
+}
+
+const char *cumodp_last_error(void) { return g_last_error.c_str(); }
+
+cumodp_err cumodp_fftmul_uni_dev(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd, sfixn dg,
+                                 const sfixn *Gd, sfixn p, void *stream) {
+    if (df < 0 || dg < 0 || dh < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative degree");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const int64_t s = (int64_t)df + dg + 1;
+    int e = ceil_log2(s);
+    if (e < 4) e = 4;
+    const int64_t n = (int64_t)1 << e;
+    const int64_t nout = (int64_t)dh + 1 < n ? (int64_t)dh + 1 : n;
+    return mul_dev_impl((u32 *)Hd, nout, (const u32 *)Fd, df + 1, (const u32 *)Gd, dg + 1,
+                        (u32)p, as_stream(stream));
+}
+
+cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
+                                  const sfixn *G, sfixn *H, sfixn p, void *stream) {
+    if (count < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative count");
+    if (count == 0) return CUMODP_SUCCESS;
+    std::vector<MulDesc> d((size_t)count);
+    int emax = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        const cumodp_mul_item &it = items[i];
+        if (it.df < 0 || it.dg < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative degree");
+        int e = ceil_log2((int64_t)it.df + it.dg + 1);
+        if (e < 4) e = 4;
+        emax = e > emax ? e : emax;
+        d[(size_t)i].f = it.f_off; d[(size_t)i].g = it.g_off; d[(size_t)i].h = it.h_off;
+        d[(size_t)i].lf = it.df + 1; d[(size_t)i].lg = it.dg + 1;
+        d[(size_t)i].lh = it.df + it.dg + 1; d[(size_t)i].logn = e;
+    }
+    cumodp_err ce = check_prime_size((u32)p, emax);
+    if (ce != CUMODP_SUCCESS) return ce;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const Family *fam = family_default((u32)p);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    hipError_t he = launch_list_mul(d.data(), count, (const u32 *)F, (const u32 *)G, (u32 *)H, fam,
+                                    as_stream(stream));
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "list multiply", he);
+    return CUMODP_SUCCESS;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// C++ linkage drop-ins
+// ===========================================================================
+void stockham_dev(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
+    (void)n;
+    report("stockham_dev", ntt_dev_impl((u32 *)X, 1, k, (u32)w, (u32)p, false, 0));
+}
+
+void stockham_dev(sfixn *X, sfixn n, sfixn k, const sfixn *W, sfixn p) {
+    stockham_dev(X, n, k, (sfixn)read_root(W), p);
+}
+
+void inverse_stockham_dev(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
+    (void)n;
+    report("inverse_stockham_dev", ntt_dev_impl((u32 *)X, 1, k, (u32)w, (u32)p, true, 0));
+}
+
+void inverse_stockham_dev(sfixn *X, sfixn n, sfixn k, const sfixn *W, sfixn p) {
+    // W holds the powers of the inverse root: DFT with W[1], then scaling
+    u32 wi = read_root(W);
+    inverse_stockham_dev(X, n, k, (sfixn)cm_host_inv(wi % (u32)p, (u32)p), p);
+}
+
+void stockham_host(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
+    (void)n;
+    report("stockham_host", ntt_host_impl(X, 1, k, (u32)w, (u32)p, false));
+}
+
+void inverse_stockham_host(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
+    (void)n;
+    report("inverse_stockham_host", ntt_host_impl(X, 1, k, (u32)w, (u32)p, true));
+}
+
+void stockham_poly_mul_dev(sfixn n, sfixn e, sfixn *F, sfixn *G, sfixn p) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    cumodp_err ce = check_prime_size((u32)p, e);
+    if (ce == CUMODP_SUCCESS && e < 4) ce = fail(CUMODP_FFT_SIZE_TOO_SMALL, "transform size below 16");
+    if (ce == CUMODP_SUCCESS) {
+        // cyclic product of length n = 2^e, as the reference computes it
+        const Family *fam = family_default((u32)p);
+        hipError_t he = hipErrorOutOfMemory;
+        if (fam && e <= kMaxLdsLog) {
+            MulDesc d;
+            d.f = 0; d.g = 0; d.h = 0;
+            d.lf = d.lg = d.lh = (int)n; d.logn = e;
+            he = launch_mul_lds_one(e, (const u32 *)F, (const u32 *)G, (u32 *)F, nullptr, 1, d,
+                                    fam, 0);
+        } else if (fam) {
+            he = launch_mul_four_step(e, (u32 *)F, (u32 *)G, 1, fam, 0);
+        }
+        if (he == hipSuccess) he = hipStreamSynchronize(0);
+        if (he != hipSuccess) ce = fail(CUMODP_FFT_ERROR, "multiply", he);
+    }
+    report("stockham_poly_mul_dev", ce);
+}
+
+void stockham_poly_mul_host(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg,
+                            const sfixn *G, sfixn p) {
+    report("stockham_poly_mul_host", mul_host_impl(dh, H, df, F, dg, G, p));
+}
+
+void list_stockham_host(sfixn *X, sfixn m, sfixn k, sfixn w, sfixn p) {
+    report("list_stockham_host", ntt_host_impl(X, m, k, (u32)w, (u32)p, false));
+}
+
+void list_stockham_dev(sfixn *X, sfixn m, sfixn k, sfixn w, sfixn p) {
+    report("list_stockham_dev", ntt_dev_impl((u32 *)X, m, k, (u32)w, (u32)p, false, 0));
+}
+
+void list_stockham_dev(sfixn *X, sfixn m, sfixn k, const sfixn *W, sfixn p) {
+    list_stockham_dev(X, m, k, (sfixn)read_root(W), p);
+}
+
+void list_inv_stockham_host(sfixn *X, sfixn m, sfixn k, sfixn w, sfixn p) {
+    // src/list_inv_stockham.cu:71-88: forward with w, forward with w^-1,
+    // scaling by n^-1 (net: identity for a primitive root w)
+    cumodp_err ce = ntt_host_impl(X, m, k, (u32)w, (u32)p, false);
+    if (ce == CUMODP_SUCCESS) ce = ntt_host_impl(X, m, k, cm_host_inv((u32)w % (u32)p, (u32)p), (u32)p, false);
+    if (ce == CUMODP_SUCCESS) {
+        u32 ni = cm_host_inv(cm_host_pow(2, k, (u32)p), (u32)p);
+        for (int64_t i = 0; i < ((int64_t)m << k); ++i) X[i] = (sfixn)cm_host_mul((u32)X[i], ni, (u32)p);
+    }
+    report("list_inv_stockham_host", ce);
+}
+
+void get_powers_binary(sfixn e, sfixn *W, sfixn w, sfixn p) {
+    hipError_t he = launch_powers((u32 *)W, (int64_t)1 << e, (u32)w, (u32)p, 0);
+    if (he == hipSuccess) he = hipStreamSynchronize(0);
+    if (he != hipSuccess) report("get_powers_binary", fail(CUMODP_FFT_ERROR, "powers", he));
+}
+
+void pointwise_mul_dev(sfixn n, sfixn k, sfixn *Y, const sfixn *X, sfixn p) {
+    (void)k;
+    hipError_t he = launch_pointwise((u32 *)Y, (const u32 *)X, n, (u32)p, 0);
+    if (he == hipSuccess) he = hipStreamSynchronize(0);
+    if (he != hipSuccess) report("pointwise_mul_dev", fail(CUMODP_FFT_ERROR, "pointwise", he));
+}
+
+void scale_vector_dev(sfixn a, sfixn n, sfixn *X, sfixn p) {
+    hipError_t he = launch_scale((u32 *)X, n, (u32)a, (u32)p, 0);
+    if (he == hipSuccess) he = hipStreamSynchronize(0);
+    if (he != hipSuccess) report("scale_vector_dev", fail(CUMODP_FFT_ERROR, "scale", he));
+}
+
+void expand_to_fft_dev(sfixn m, sfixn n, sfixn *X) {
+    hipError_t he = launch_fill((u32 *)X + n, (int64_t)m - n, 0u, 0);
+    if (he == hipSuccess) he = hipStreamSynchronize(0);
+    if (he != hipSuccess) report("expand_to_fft_dev", fail(CUMODP_FFT_ERROR, "fill", he));
+}
+
+void reset_vector_dev(sfixn n, sfixn *X) {
+    hipError_t he = launch_fill((u32 *)X, n, 0u, 0);
+    if (he == hipSuccess) he = hipStreamSynchronize(0);
+    if (he != hipSuccess) report("reset_vector_dev", fail(CUMODP_FFT_ERROR, "fill", he));
+}
+
+void bivariate_stockham_dev(sfixn *X, sfixn em, const sfixn *Wm, sfixn en, const sfixn *Wn,
+                            sfixn p) {
+    report("bivariate_stockham_dev",
+           biv_dev_impl((u32 *)X, em, read_root(Wm), en, read_root(Wn), (u32)p, false, 0));
+}
+
+void inverse_bivariate_stockham_dev(sfixn *X, sfixn em, const sfixn *Wm, sfixn en,
+                                    const sfixn *Wn, sfixn p) {
+    // src/list_stockham.cu:1251-1263: the tables already hold inverse roots
+    const u32 P = (u32)p;
+    report("inverse_bivariate_stockham_dev",
+           biv_dev_impl((u32 *)X, em, cm_host_inv(read_root(Wm) % P, P), en,
+                        cm_host_inv(read_root(Wn) % P, P), P, true, 0));
+}
+
+void bivariate_stockham_host(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, sfixn p) {
+    report("bivariate_stockham_host", biv_host_impl(X, em, (u32)wm, en, (u32)wn, (u32)p, false));
+}
+
+void inverse_bivariate_stockham_host(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, sfixn p) {
+    report("inverse_bivariate_stockham_host",
+           biv_host_impl(X, em, (u32)wm, en, (u32)wn, (u32)p, true));
+}
+
+void bi_stockham_poly_mul_dev(sfixn m, sfixn em, sfixn n, sfixn en, sfixn *F, sfixn *G,
+                              sfixn p) {
+    // src/list_stockham.cu:1317-1348: F <- DFT^-1(DFT(F) DFT(G)), G overwritten
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const u32 P = (u32)p;
+    const u32 wm = default_root(P, em), wn = default_root(P, en);
+    cumodp_err ce = (wm && wn) ? CUMODP_SUCCESS
+                               : fail(CUMODP_FOURIER_DEGREE_TOO_SMALL, "2^k does not divide p-1");
+    if (ce == CUMODP_SUCCESS) ce = biv_dev_impl((u32 *)F, em, wm, en, wn, P, false, 0);
+    if (ce == CUMODP_SUCCESS) ce = biv_dev_impl((u32 *)G, em, wm, en, wn, P, false, 0);
+    if (ce == CUMODP_SUCCESS) {
+        hipError_t he = launch_pointwise((u32 *)F, (const u32 *)G, (int64_t)m * n, P, 0);
+        if (he != hipSuccess) ce = fail(CUMODP_FFT_ERROR, "pointwise", he);
+    }
+    if (ce == CUMODP_SUCCESS) ce = biv_dev_impl((u32 *)F, em, wm, en, wn, P, true, 0);
+    report("bi_stockham_poly_mul_dev", ce);
+}

@@ -1,0 +1,263 @@
+// batch.hip -- one call multiplying many pairs of uneven degree mod p.
+//
+// The reference has no list multiply for uneven degrees: list_stockham_dev
+// (src/list_stockham.cu:595-630) transforms m rows of one length, and the
+// unbuilt draft src/.list_fft_poly_mul.cu:15-53 multiplies equal-length
+// pairs.  Here products are binned by transform size 2^e:
+//   * e <= 12: one fused launch for every such product.  Each 256-thread
+//     workgroup takes a 4096-word tile holding 4096 / 2^e products of one
+//     size and runs the whole multiply in LDS; the tile's size selects the
+//     code path at run time, so one launch covers all sizes;
+//   * e == 13: one launch of the 8192-point workgroup multiply;
+//   * e >= 14: per size, the inputs are gathered into zero-padded slabs,
+//     multiplied by the batched four-step kernels and scattered back.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "ntt_core.h"
+
+namespace cm {
+
+struct TileDesc {
+    int32_t first;   // first descriptor of the tile
+    int32_t count;   // products in the tile (<= 4096 >> logn)
+    int32_t logn;
+    int32_t pad;
+};
+
+constexpr int kMixT = 256;          // threads of the mixed-size workgroup
+constexpr int kMixWords = 4096;     // words per tile
+
+template <int LOGL>
+using MixTile = Tile<LOGL, (kMixWords >> LOGL), false, kMixT>;
+
+template <int LOGL>
+CM_DEV void mul_tile(u32 *lds, const u32 *F, const u32 *G, u32 *H, const MulDesc *d, int cnt,
+                     const uint2 *tw, const uint2 *twi, int LT, u32 p, u32 pinv, uint2 sc) {
+    using TL = MixTile<LOGL>;
+    using LR = LastRound<TL>;
+    constexpr int L = TL::L, C = TL::C;
+    const int tws = LT - LOGL;
+    // element idx of the tile: vector c = idx >> LOGL, position i
+    for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
+        const int c = idx >> LOGL, i = idx & (L - 1);
+        u32 v = 0;
+        if (c < cnt && i < d[c].lf) v = F[d[c].f + i];
+        lds[TL::addr(c, i)] = v;
+    }
+    __syncthreads();
+    cm_dif_rounds<TL, 0, TL::NR - 1>(lds, tw, tws, p);
+    u32 fr[LR::GPT][1 << LR::R];
+    LR::dif_keep(lds, fr, tw, tws, p);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
+        const int c = idx >> LOGL, i = idx & (L - 1);
+        u32 v = 0;
+        if (c < cnt && i < d[c].lg) v = G[d[c].g + i];
+        lds[TL::addr(c, i)] = v;
+    }
+    __syncthreads();
+    cm_dif_rounds<TL, 0, TL::NR - 1>(lds, tw, tws, p);
+    LR::dif_mul_dit(lds, fr, tw, twi, tws, p, pinv, sc);
+    cm_dit_rounds<TL, TL::NR - 2, 0>(lds, twi, tws, p);
+    for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
+        const int c = idx >> LOGL, i = idx & (L - 1);
+        if (c < cnt && i < d[c].lh) H[d[c].h + i] = lds[TL::addr(c, i)];
+    }
+    (void)C;
+}
+
+__global__ __launch_bounds__(kMixT) void k_mul_mixed(
+    const u32 *__restrict__ F, const u32 *__restrict__ G, u32 *__restrict__ H,
+    const MulDesc *__restrict__ descs, const TileDesc *__restrict__ tiles,
+    const uint2 *__restrict__ tw, const uint2 *__restrict__ twi, int LT, u32 p, u32 pinv,
+    const uint2 *__restrict__ scales) {
+    __shared__ u32 lds[kMixWords + (kMixWords >> 4)];
+    const TileDesc t = tiles[blockIdx.x];
+    const MulDesc *d = descs + t.first;
+    const uint2 sc = scales[t.logn];
+    switch (t.logn) {
+    case 4: mul_tile<4>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 5: mul_tile<5>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 6: mul_tile<6>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 7: mul_tile<7>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 8: mul_tile<8>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 9: mul_tile<9>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 10: mul_tile<10>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 11: mul_tile<11>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 12: mul_tile<12>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    default: break;
+    }
+}
+
+// zero-padded copy of operand `which` (0: F, 1: G) of descriptor row y into
+// row y of a slab of n-word rows
+__global__ void k_gather(u32 *__restrict__ slab, int64_t n, const u32 *__restrict__ src,
+                         const MulDesc *__restrict__ d, int which) {
+    const MulDesc dd = d[blockIdx.y];
+    const int64_t off = which ? dd.g : dd.f;
+    const int len = which ? dd.lg : dd.lf;
+    u32 *row = slab + (int64_t)blockIdx.y * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        row[i] = i < len ? src[off + i] : 0u;
+}
+
+__global__ void k_scatter(u32 *__restrict__ H, const u32 *__restrict__ slab, int64_t n,
+                          const MulDesc *__restrict__ d) {
+    const MulDesc dd = d[blockIdx.y];
+    const u32 *row = slab + (int64_t)blockIdx.y * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dd.lh;
+         i += (int64_t)gridDim.x * blockDim.x)
+        H[dd.h + i] = row[i];
+}
+
+namespace {
+
+// Pinned staging for descriptors; reused once the previous upload finished.
+struct Staging {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+
+Staging &staging() {
+    static Staging s;
+    return s;
+}
+
+hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s) {
+    Staging &st = staging();
+    hipError_t e;
+    if (!st.done) {
+        e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    } else {
+        e = hipEventSynchronize(st.done);
+        if (e != hipSuccess) return e;
+    }
+    if (st.cap < bytes) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.cap = 0;
+        e = hipHostMalloc(&st.host, bytes);
+        if (e != hipSuccess) return e;
+        st.cap = bytes;
+    }
+    memcpy(st.host, src, bytes);
+    e = hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipEventRecord(st.done, s);
+}
+
+const uint2 *scale_table(const Family *fam) {
+    // 2^32 / 2^e mod p for e = 0 .. 13, one device table per family
+    static std::mutex m;
+    static std::vector<std::pair<const Family *, uint2 *>> tabs;
+    std::lock_guard<std::mutex> lk(m);
+    for (auto &t : tabs)
+        if (t.first == fam) return t.second;
+    uint2 h[16];
+    for (int e = 0; e < 16; ++e) {
+        u32 ninv = cm_host_inv(cm_host_pow(2, e, fam->p), fam->p);
+        u32 c = cm_host_mul(cm_host_r1(fam->p), ninv, fam->p);
+        h[e] = make_uint2(c, cm_host_shoup_q(c, fam->p));
+    }
+    uint2 *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(h)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    tabs.push_back({fam, d});
+    return d;
+}
+
+}  // namespace
+
+hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, const u32 *G,
+                           u32 *H, const Family *fam, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    // order: small sizes (tiled), then 2^13, then four-step classes by size
+    std::vector<int64_t> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return descs[a].logn < descs[b].logn; });
+    std::vector<MulDesc> sorted((size_t)count);
+    for (int64_t i = 0; i < count; ++i) sorted[(size_t)i] = descs[order[(size_t)i]];
+    std::vector<TileDesc> tiles;
+    int64_t i = 0;
+    while (i < count && sorted[(size_t)i].logn <= 12) {
+        const int lg = sorted[(size_t)i].logn;
+        const int per = kMixWords >> lg;
+        int64_t j = i;
+        while (j < count && sorted[(size_t)j].logn == lg && j - i < per) ++j;
+        tiles.push_back(TileDesc{(int32_t)i, (int32_t)(j - i), lg, 0});
+        i = j;
+    }
+    const int64_t n_small_end = i;
+    int64_t n13_end = n_small_end;
+    while (n13_end < count && sorted[(size_t)n13_end].logn == 13) ++n13_end;
+
+    const size_t desc_bytes = sizeof(MulDesc) * (size_t)count;
+    const size_t tile_bytes = sizeof(TileDesc) * tiles.size();
+    std::vector<char> blob(desc_bytes + tile_bytes);
+    memcpy(blob.data(), sorted.data(), desc_bytes);
+    if (tile_bytes) memcpy(blob.data() + desc_bytes, tiles.data(), tile_bytes);
+    char *dblob = (char *)workspace(7, blob.size());
+    if (!dblob) return hipErrorOutOfMemory;
+    hipError_t e = stage_upload(blob.data(), blob.size(), dblob, s);
+    if (e != hipSuccess) return e;
+    const MulDesc *d_descs = (const MulDesc *)dblob;
+    const TileDesc *d_tiles = (const TileDesc *)(dblob + desc_bytes);
+
+    if (!tiles.empty()) {
+        const uint2 *sc = scale_table(fam);
+        if (!sc) return hipErrorOutOfMemory;
+        ProfScope ps(kProfMulMixed, s);
+        hipLaunchKernelGGL(k_mul_mixed, dim3((unsigned)tiles.size()), dim3(kMixT), 0, s, F, G, H,
+                           d_descs, d_tiles, fam->fwd, fam->inv, fam->LT, fam->p, fam->pinv, sc);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n13_end > n_small_end) {
+        e = launch_mul_lds(13, F, G, H, d_descs + n_small_end, (int)(n13_end - n_small_end), fam,
+                           s);
+        if (e != hipSuccess) return e;
+    }
+    for (int64_t a = n13_end; a < count;) {
+        const int lg = sorted[(size_t)a].logn;
+        int64_t b = a;
+        while (b < count && sorted[(size_t)b].logn == lg) ++b;
+        const int64_t n = (int64_t)1 << lg;
+        // process the class in chunks that keep the slabs within 1 GiB each
+        const int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 28) / n);
+        for (int64_t c0 = a; c0 < b; c0 += chunk) {
+            const int cnt = (int)std::min<int64_t>(chunk, b - c0);
+            u32 *sf = (u32 *)workspace(5, sizeof(u32) * (size_t)(n * cnt));
+            u32 *sg = (u32 *)workspace(6, sizeof(u32) * (size_t)(n * cnt));
+            if (!sf || !sg) return hipErrorOutOfMemory;
+            const unsigned gx = (unsigned)std::min<int64_t>(n / 1024, 64);
+            {
+            ProfScope ps(kProfGatherScatter, s);
+            hipLaunchKernelGGL(k_gather, dim3(gx, cnt), dim3(1024), 0, s, sf, n, F,
+                               d_descs + c0, 0);
+            hipLaunchKernelGGL(k_gather, dim3(gx, cnt), dim3(1024), 0, s, sg, n, G,
+                               d_descs + c0, 1);
+            }
+            e = hipGetLastError();
+            if (e == hipSuccess) e = launch_mul_four_step(lg, sf, sg, cnt, fam, s);
+            if (e != hipSuccess) return e;
+            {
+            ProfScope ps(kProfGatherScatter, s);
+            hipLaunchKernelGGL(k_scatter, dim3(gx, cnt), dim3(1024), 0, s, H, sf, n,
+                               d_descs + c0);
+            }
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        a = b;
+    }
+    return hipSuccess;
+}
+
+}  // namespace cm

@@ -1,0 +1,72 @@
+/* cumodp_amd.h -- C-ABI entry points of the MI355X library that have no
+ * single counterpart in the reference.  They serve callers that keep data in
+ * device memory (no per-call host copies), batch many products, or need the
+ * integer multiplier without GMP/thrust types.  Plain pointers and sizes only;
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream). */
+#ifndef CUMODP_AMD_H
+#define CUMODP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device memory and synchronisation (for callers without HIP headers) */
+CUMODP_API void *cumodp_dev_alloc(size_t bytes);
+CUMODP_API cumodp_err cumodp_dev_free(void *ptr);
+CUMODP_API cumodp_err cumodp_memcpy_h2d(void *dst, const void *src, size_t bytes);
+CUMODP_API cumodp_err cumodp_memcpy_d2h(void *dst, const void *src, size_t bytes);
+CUMODP_API cumodp_err cumodp_memcpy_d2d(void *dst, const void *src, size_t bytes);
+CUMODP_API cumodp_err cumodp_synchronize(void);
+CUMODP_API void *cumodp_stream_create(void);
+CUMODP_API cumodp_err cumodp_stream_destroy(void *stream);
+CUMODP_API cumodp_err cumodp_stream_synchronize(void *stream);
+/* error string of the last failing device call in this thread ("" if none) */
+CUMODP_API const char *cumodp_last_error(void);
+
+/* ---- univariate multiply on device buffers --------------------------------
+ * Device-resident form of stockham_poly_mul_host (src/stockham.cu:562-611):
+ * Hd[0 .. min(dh+1, n)) = (Fd * Gd mod p), n = 2^ceil(log2(df+dg+1)), any
+ * n >= 2 (no 256 cut-off).  Fd/Gd are read only and hold df+1 / dg+1 words;
+ * Hd may alias Fd or Gd.  Scratch space is owned by the library.  Stream
+ * ordered (returns before the product is complete). */
+CUMODP_API cumodp_err cumodp_fftmul_uni_dev(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd, sfixn dg,
+                                 const sfixn *Gd, sfixn p, void *stream);
+
+/* ---- list multiply of pairs of uneven degree ------------------------------
+ * One call multiplies `count` independent pairs mod p.  Item i reads
+ * F[f_off .. f_off+df], G[g_off .. g_off+dg] and writes H[h_off ..
+ * h_off+df+dg] (df+dg+1 words).  Output ranges must not overlap each other
+ * or any input.  Products are binned by transform size; sizes up to 2^13 run
+ * many products per workgroup in one launch per size class, larger ones as
+ * batched four-step transforms.  Replaces a loop of stockham_poly_mul_dev
+ * calls; the reference's list path (src/list_stockham.cu:595-630) only
+ * transforms rows of one length. */
+typedef struct {
+    int64_t f_off, g_off, h_off;
+    int32_t df, dg;
+} cumodp_mul_item;
+
+CUMODP_API cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
+                                  const sfixn *G, sfixn *H, sfixn p, void *stream);
+
+/* ---- optional per-kernel timing ------------------------------------------
+ * Off unless enabled here or with CUMODP_PROFILE=1.  When on, each hot kernel
+ * launch is bracketed by HIP events on its stream; cumodp_profile_read
+ * synchronises them and returns the total milliseconds and launch count of
+ * kernel `id` (names from cumodp_profile_name, ids 0 .. cumodp_profile_count-1). */
+CUMODP_API cumodp_err cumodp_profile_enable(int on);
+CUMODP_API void cumodp_profile_reset(void);
+CUMODP_API int cumodp_profile_count(void);
+CUMODP_API const char *cumodp_profile_name(int id);
+CUMODP_API cumodp_err cumodp_profile_read(int id, double *total_ms, int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
