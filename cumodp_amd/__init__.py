@@ -127,6 +127,8 @@ def lib():
         L.cumodp_profile_name.restype = C.c_char_p
         L.cumodp_profile_read.argtypes = [s, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.cumodp_profile_read.restype = err
+        L.cumodp_tune_set_split.argtypes = [s, s]
+        L.cumodp_tune_set_split.restype = err
         _LIB = L
     return _LIB
 

@@ -65,16 +65,14 @@ cumodp_err mul_dev_impl(u32 *Hd, int64_t nout, const u32 *Fd, int64_t lf, const 
         d.lf = (int)lf; d.lg = (int)lg; d.lh = (int)(nout < n ? nout : n); d.logn = e;
         he = launch_mul_lds_one(e, Fd, Gd, Hd, nullptr, 1, d, fam, s);
     } else {
+        // the column kernels read the operands with zero padding and write
+        // the first nout product words straight to Hd
         u32 *wf = (u32 *)workspace(0, sizeof(u32) * (size_t)n);
         u32 *wg = (u32 *)workspace(1, sizeof(u32) * (size_t)n);
         if (!wf || !wg) return fail(CUMODP_OUT_OF_MEMORY, "multiply workspace");
-        he = hipMemcpyAsync(wf, Fd, sizeof(u32) * (size_t)lf, hipMemcpyDeviceToDevice, s);
-        if (he == hipSuccess) he = launch_fill(wf + lf, n - lf, 0, s);
-        if (he == hipSuccess) he = hipMemcpyAsync(wg, Gd, sizeof(u32) * (size_t)lg, hipMemcpyDeviceToDevice, s);
-        if (he == hipSuccess) he = launch_fill(wg + lg, n - lg, 0, s);
-        if (he == hipSuccess) he = launch_mul_four_step(e, wf, wg, 1, fam, s);
         const int64_t nw = nout < n ? nout : n;
-        if (he == hipSuccess) he = hipMemcpyAsync(Hd, wf, sizeof(u32) * (size_t)nw, hipMemcpyDeviceToDevice, s);
+        FourStepIO io{Fd, Gd, lf, lg, 0, wf, wg, Hd, nw, 0, 1};
+        he = launch_mul_four_step(e, io, fam, s);
     }
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "multiply", he);
     return CUMODP_SUCCESS;
@@ -122,7 +120,7 @@ cumodp_err ntt_dev_impl(u32 *X, int64_t m, int k, u32 w, u32 p, bool inverse, hi
             u32 ni = cm_host_inv(cm_host_pow(2, k, p), p);
             sc = make_uint2(ni, cm_host_shoup_q(ni, p));
         }
-        he = launch_ntt_lds(k, X, m, inverse ? fam->inv : fam->fwd, sc, fam, s);
+        he = launch_ntt_lds(k, X, m, inverse, sc, fam, s);
     } else {
         const int64_t n = (int64_t)1 << k;
         if (m > INT_MAX) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "too many rows");
@@ -168,7 +166,7 @@ cumodp_err biv_dev_impl(u32 *X, int em, u32 wm, int en, u32 wn, u32 p, bool inve
         u32 ni = cm_host_inv(cm_host_pow(2, em + en, p), p);
         sc = make_uint2(ni, cm_host_shoup_q(ni, p));
     }
-    hipError_t he = launch_col_ntt(em, X, (int64_t)1 << en, fm->fwd, sc, fm, s);
+    hipError_t he = launch_col_ntt(em, X, (int64_t)1 << en, sc, fm, s);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "bivariate transform", he);
     return CUMODP_SUCCESS;
@@ -359,6 +357,11 @@ cumodp_err cumodp_fftmul_uni_dev(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd,
                         (u32)p, as_stream(stream));
 }
 
+cumodp_err cumodp_tune_set_split(int k, int k2) {
+    return set_four_step_split(k, k2) ? CUMODP_SUCCESS
+                                      : fail(CUMODP_ASSUMPTION_ERROR, "unsupported four-step split");
+}
+
 cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
                                   const sfixn *G, sfixn *H, sfixn p, void *stream) {
     if (count < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative count");
@@ -436,7 +439,9 @@ void stockham_poly_mul_dev(sfixn n, sfixn e, sfixn *F, sfixn *G, sfixn p) {
             he = launch_mul_lds_one(e, (const u32 *)F, (const u32 *)G, (u32 *)F, nullptr, 1, d,
                                     fam, 0);
         } else if (fam) {
-            he = launch_mul_four_step(e, (u32 *)F, (u32 *)G, 1, fam, 0);
+            FourStepIO io{(const u32 *)F, (const u32 *)G, n, n, 0, (u32 *)F, (u32 *)G, (u32 *)F,
+                          n, 0, 1};
+            he = launch_mul_four_step(e, io, fam, 0);
         }
         if (he == hipSuccess) he = hipStreamSynchronize(0);
         if (he != hipSuccess) ce = fail(CUMODP_FFT_ERROR, "multiply", he);

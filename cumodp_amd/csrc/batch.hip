@@ -34,60 +34,60 @@ template <int LOGL>
 using MixTile = Tile<LOGL, (kMixWords >> LOGL), false, kMixT>;
 
 template <int LOGL>
-CM_DEV void mul_tile(u32 *lds, const u32 *F, const u32 *G, u32 *H, const MulDesc *d, int cnt,
-                     const uint2 *tw, const uint2 *twi, int LT, u32 p, u32 pinv, uint2 sc) {
+CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const MulDesc *d,
+                     int cnt, const uint2 *lv, const uint2 *lvi, u32 p, u32 pinv, uint2 sc) {
     using TL = MixTile<LOGL>;
-    using LR = LastRound<TL>;
-    constexpr int L = TL::L, C = TL::C;
-    const int tws = LT - LOGL;
-    // element idx of the tile: vector c = idx >> LOGL, position i
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
-        const int c = idx >> LOGL, i = idx & (L - 1);
-        u32 v = 0;
-        if (c < cnt && i < d[c].lf) v = F[d[c].f + i];
-        lds[TL::addr(c, i)] = v;
+    constexpr int L = TL::L;
+    // element idx of the tile: vector c = idx >> LOGL, position i; all loads
+    // of a thread are issued before the first LDS store
+    constexpr int PER = TL::WORDS / kMixT;
+    u32 fa[PER], ga[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * kMixT, c = idx >> LOGL, i = idx & (L - 1);
+        u32 f = 0, g = 0;
+        if (c < cnt) {
+            if (i < d[c].lf) f = F[d[c].f + i];
+            if (i < d[c].lg) g = G[d[c].g + i];
+        }
+        fa[r] = f;
+        ga[r] = g;
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * kMixT, c = idx >> LOGL, i = idx & (L - 1);
+        A[TL::addr(c, i)] = fa[r];
+        B[TL::addr(c, i)] = ga[r];
     }
     __syncthreads();
-    cm_dif_rounds<TL, 0, TL::NR - 1>(lds, tw, tws, p);
-    u32 fr[LR::GPT][1 << LR::R];
-    LR::dif_keep(lds, fr, tw, tws, p);
-    __syncthreads();
+    cm_tile_multiply<TL>(A, B, lv, lvi, p, pinv, sc);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
         const int c = idx >> LOGL, i = idx & (L - 1);
-        u32 v = 0;
-        if (c < cnt && i < d[c].lg) v = G[d[c].g + i];
-        lds[TL::addr(c, i)] = v;
+        if (c < cnt && i < d[c].lh) H[d[c].h + i] = A[TL::addr(c, i)];
     }
-    __syncthreads();
-    cm_dif_rounds<TL, 0, TL::NR - 1>(lds, tw, tws, p);
-    LR::dif_mul_dit(lds, fr, tw, twi, tws, p, pinv, sc);
-    cm_dit_rounds<TL, TL::NR - 2, 0>(lds, twi, tws, p);
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
-        const int c = idx >> LOGL, i = idx & (L - 1);
-        if (c < cnt && i < d[c].lh) H[d[c].h + i] = lds[TL::addr(c, i)];
-    }
-    (void)C;
 }
 
 __global__ __launch_bounds__(kMixT) void k_mul_mixed(
     const u32 *__restrict__ F, const u32 *__restrict__ G, u32 *__restrict__ H,
     const MulDesc *__restrict__ descs, const TileDesc *__restrict__ tiles,
-    const uint2 *__restrict__ tw, const uint2 *__restrict__ twi, int LT, u32 p, u32 pinv,
+    const uint2 *__restrict__ lv, const uint2 *__restrict__ lvi, u32 p, u32 pinv,
     const uint2 *__restrict__ scales) {
-    __shared__ u32 lds[kMixWords + (kMixWords >> 4)];
+    constexpr int W = kMixWords + (kMixWords >> 4);
+    __shared__ u32 lds[2 * W];
+    u32 *A = lds, *B = lds + W;
     const TileDesc t = tiles[blockIdx.x];
     const MulDesc *d = descs + t.first;
     const uint2 sc = scales[t.logn];
     switch (t.logn) {
-    case 4: mul_tile<4>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 5: mul_tile<5>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 6: mul_tile<6>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 7: mul_tile<7>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 8: mul_tile<8>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 9: mul_tile<9>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 10: mul_tile<10>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 11: mul_tile<11>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
-    case 12: mul_tile<12>(lds, F, G, H, d, t.count, tw, twi, LT, p, pinv, sc); break;
+    case 4: mul_tile<4>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 5: mul_tile<5>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 6: mul_tile<6>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 7: mul_tile<7>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 8: mul_tile<8>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 9: mul_tile<9>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 10: mul_tile<10>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 11: mul_tile<11>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 12: mul_tile<12>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
     default: break;
     }
 }
@@ -215,7 +215,7 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
         if (!sc) return hipErrorOutOfMemory;
         ProfScope ps(kProfMulMixed, s);
         hipLaunchKernelGGL(k_mul_mixed, dim3((unsigned)tiles.size()), dim3(kMixT), 0, s, F, G, H,
-                           d_descs, d_tiles, fam->fwd, fam->inv, fam->LT, fam->p, fam->pinv, sc);
+                           d_descs, d_tiles, fam->lv, fam->lvi, fam->p, fam->pinv, sc);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -245,7 +245,8 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
                                d_descs + c0, 1);
             }
             e = hipGetLastError();
-            if (e == hipSuccess) e = launch_mul_four_step(lg, sf, sg, cnt, fam, s);
+            FourStepIO io{sf, sg, n, n, n, sf, sg, sf, n, n, cnt};
+            if (e == hipSuccess) e = launch_mul_four_step(lg, io, fam, s);
             if (e != hipSuccess) return e;
             {
             ProfScope ps(kProfGatherScatter, s);

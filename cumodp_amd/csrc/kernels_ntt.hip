@@ -3,13 +3,15 @@
 // Replaces the reference's one-stage-per-launch radix-2 Stockham passes
 // (src/stockham.cu:75-397, src/list_stockham.cu:108-576, src/fft_aux.cu:70-322)
 // with:
-//   * k_mul_lds   -- a whole multiply (forward F, forward G, pointwise,
+//   * k_mul_lds   -- a whole multiply (forward F and G, pointwise product,
 //                    inverse, scaling) inside one workgroup for 2^k <= 2^13;
 //   * k_col_fwd / k_row_mul / k_col_inv -- a three-kernel four-step multiply
-//                    for larger sizes: column transforms with the outer
-//                    twiddle fused, fused row transforms + pointwise product +
-//                    inverse row transforms, inverse column transforms;
-//   * k_ntt_lds / k_row_out -- natural-order stand-alone transforms.
+//                    for larger sizes: column transforms of both operands with
+//                    the zero-padding gather fused; row pass with the outer
+//                    twiddles, both row transforms, the pointwise product and
+//                    the inverse row transform fused; inverse column
+//                    transforms with the truncating store fused;
+//   * k_ntt_lds / k_row_out / k_col_ntt -- natural-order transforms.
 #include "internal.h"
 #include "ntt_core.h"
 
@@ -26,16 +28,16 @@ CM_DEV int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // whole multiply in one workgroup (C products per workgroup)
 // ---------------------------------------------------------------------------
+// DESC: descriptors from `descs` (or the by-value `one` when descs is null);
+// otherwise row d of F, G, H at offset d * L, full length.  H may alias F.
 template <int LOGL, int C, bool DESC>
 __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
-    const u32 *F, const u32 *__restrict__ G, u32 *H,  // H may alias F
-    const MulDesc *__restrict__ descs, int ndesc, MulDesc one, const uint2 *__restrict__ tw,
-    const uint2 *__restrict__ twi, int tws, u32 p, u32 pinv, uint2 sc) {
-    // DESC: descriptors from `descs` (or the by-value `one` when descs is
-    // null); otherwise row d of F, G, H at offset d * L, full length.
+    const u32 *F, const u32 *__restrict__ G, u32 *H, const MulDesc *__restrict__ descs,
+    int ndesc, MulDesc one, const uint2 *__restrict__ lv, const uint2 *__restrict__ lvi, u32 p,
+    u32 pinv, uint2 sc) {
     using TL = Tile<LOGL, C, false>;
-    using LR = LastRound<TL>;
-    __shared__ u32 lds[TL::LDS_WORDS];
+    __shared__ u32 lds[2 * TL::LDS_WORDS];
+    u32 *A = lds, *B = lds + TL::LDS_WORDS;
     const int L = TL::L;
 
     int64_t off_f[C], off_g[C], off_h[C];
@@ -57,30 +59,32 @@ __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
             lf[c] = lg[c] = lh[c] = d < ndesc ? L : 0;
         }
     }
-
+    // all loads of a thread are issued before the first LDS store
+    constexpr int PER = TL::WORDS / TL::T;
+    u32 fa[PER], ga[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+        fa[r] = i < lf[c] ? F[off_f[c] + i] : 0u;
+        ga[r] = i < lg[c] ? G[off_g[c] + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+        A[TL::addr(c, i)] = fa[r];
+        B[TL::addr(c, i)] = ga[r];
+    }
+    __syncthreads();
+    cm_tile_multiply<TL>(A, B, lv, lvi, p, pinv, sc);
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+        if (i < lh[c]) H[off_h[c] + i] = A[TL::addr(c, i)];
+    }
+    // words beyond the transform size (lh > L) are zero
 #pragma unroll
     for (int c = 0; c < C; ++c)
-        for (int i = threadIdx.x; i < L; i += TL::T)
-            lds[TL::addr(c, i)] = i < lf[c] ? F[off_f[c] + i] : 0u;
-    __syncthreads();
-    cm_dif_rounds<TL, 0, TL::NR - 1>(lds, tw, tws, p);
-    u32 fr[LR::GPT][1 << LR::R];
-    LR::dif_keep(lds, fr, tw, tws, p);
-    __syncthreads();
-
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-        for (int i = threadIdx.x; i < L; i += TL::T)
-            lds[TL::addr(c, i)] = i < lg[c] ? G[off_g[c] + i] : 0u;
-    __syncthreads();
-    cm_dif_rounds<TL, 0, TL::NR - 1>(lds, tw, tws, p);
-    LR::dif_mul_dit(lds, fr, tw, twi, tws, p, pinv, sc);
-    cm_dit_rounds<TL, TL::NR - 2, 0>(lds, twi, tws, p);
-
-#pragma unroll
-    for (int c = 0; c < C; ++c)
-        for (int i = threadIdx.x; i < lh[c]; i += TL::T)
-            H[off_h[c] + i] = i < L ? lds[TL::addr(c, i)] : 0u;
+        for (int i = L + threadIdx.x; i < lh[c]; i += TL::T) H[off_h[c] + i] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -88,17 +92,25 @@ __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
 // ---------------------------------------------------------------------------
 template <int LOGL, int C>
 __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_ntt_lds(
-    u32 *__restrict__ X, int64_t m, const uint2 *__restrict__ tw, int tws, u32 p, uint2 sc) {
+    u32 *__restrict__ X, int64_t m, const uint2 *__restrict__ lv, u32 p, uint2 sc) {
     using TL = Tile<LOGL, C, false>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int L = TL::L;
     const int64_t r0 = (int64_t)blockIdx.x * C;
+    constexpr int PER = TL::WORDS / TL::T;
+    u32 xa[PER];
 #pragma unroll
-    for (int c = 0; c < C; ++c)
-        for (int i = threadIdx.x; i < L; i += TL::T)
-            lds[TL::addr(c, i)] = (r0 + c < m) ? X[(r0 + c) * L + i] : 0u;
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+        xa[r] = (r0 + c < m) ? X[(r0 + c) * L + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+        lds[TL::addr(c, i)] = xa[r];
+    }
     __syncthreads();
-    cm_tile_dif<TL>(lds, tw, tws, p);
+    cm_tile_dif<TL>(lds, lv, p);
     const bool scale = sc.x != 1u;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -112,63 +124,135 @@ __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_ntt_lds(
 }
 
 // ---------------------------------------------------------------------------
-// four-step pieces: X viewed as 2^k1 rows x 2^k2 columns
+// four-step pieces: an n-word array viewed as 2^k1 rows x 2^k2 columns
 // ---------------------------------------------------------------------------
-// Forward column transforms (DIF, bit-reversed rows) followed by the outer
-// twiddle w_n^(i2 * k1); in place.  grid.x: column tiles, grid.y: arrays
-// (the first cnt0 at X0 + y*stride, the rest at X1 + (y-cnt0)*stride).
-template <int LOGL1, int C>
+// One set of arrays for the column passes: array a reads in[a*in_stride + e]
+// for e < in_len (zero beyond) and writes out[a*out_stride + e] for
+// e < out_len.
+struct ColSet {
+    const u32 *in;
+    u32 *out;
+    int64_t in_stride, in_len, out_stride, out_len;
+};
+
+// Forward column transforms (DIF, bit-reversed rows), followed by the outer
+// twiddle w_n^(i2 * k1) when OUTER (the multiply applies it in the row pass
+// instead).  grid.x: column tiles, grid.y: arrays (the first cnt0 from set
+// s0, the rest from s1).
+template <int LOGL1, int C, bool OUTER>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
-    u32 *__restrict__ X0, int cnt0, u32 *__restrict__ X1, int64_t stride, int logn2,
-    const uint2 *__restrict__ tw, int tws, const uint2 *__restrict__ olo,
-    const uint2 *__restrict__ ohi, u32 p) {
+    ColSet s0, ColSet s1, int cnt0, int logn2, const uint2 *__restrict__ lv,
+    const uint2 *__restrict__ olo, const uint2 *__restrict__ ohi, u32 p) {
     using TL = Tile<LOGL1, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int y = blockIdx.y;
-    u32 *X = y < cnt0 ? X0 + (int64_t)y * stride : X1 + (int64_t)(y - cnt0) * stride;
+    const ColSet &S = y < cnt0 ? s0 : s1;
+    const int64_t a = y < cnt0 ? y : y - cnt0;
+    const u32 *X = S.in + a * S.in_stride;
+    u32 *Y = S.out + a * S.out_stride;
     const int64_t n2 = (int64_t)1 << logn2;
     const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
-        const int i = idx / C, c = idx % C;
-        lds[cm_pad(idx)] = X[i * n2 + c0 + c];
+    constexpr int PER = TL::WORDS / TL::T;
+    u32 xa[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
+        const int64_t e = i * n2 + c0 + c;
+        xa[r] = e < S.in_len ? X[e] : 0u;
     }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
-    cm_tile_dif<TL>(lds, tw, tws, p);
+    cm_tile_dif<TL>(lds, lv, p);
     constexpr int LOM = (1 << kOuterLoBits) - 1;
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
-        const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
-        const uint2 a = olo[e & LOM], b = ohi[e >> kOuterLoBits];
-        u32 v = cm_shoup_lazy(lds[cm_pad(idx)], a.x, a.y, p);
-        v = cm_shoup(v, b.x, b.y, p);
-        X[i * n2 + c0 + c] = v;
+        u32 v = lds[cm_pad(idx)];
+        if (OUTER) {
+            const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
+            const uint2 wa = olo[e & LOM], wb = ohi[e >> kOuterLoBits];
+            v = cm_shoup(cm_shoup_lazy(v, wa.x, wa.y, p), wb.x, wb.y, p);
+        }
+        Y[i * n2 + c0 + c] = v;
     }
 }
 
-// Inverse: outer twiddle w_n^-(i2 * k1), then inverse column transforms
-// (DIT, bit-reversed rows in, natural rows out); in place.
+// Inverse column transforms (DIT, bit-reversed rows in, natural rows out)
+// with a truncating store; the inverse outer twiddle was applied by the row
+// pass.  grid.y: arrays.
 template <int LOGL1, int C>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_inv(
-    u32 *__restrict__ X0, int64_t stride, int logn2, const uint2 *__restrict__ twi, int tws,
-    const uint2 *__restrict__ oilo, const uint2 *__restrict__ oihi, u32 p) {
+    ColSet S, int logn2, const uint2 *__restrict__ lvi, u32 p) {
     using TL = Tile<LOGL1, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
-    u32 *X = X0 + (int64_t)blockIdx.y * stride;
+    const int64_t a = blockIdx.y;
+    const u32 *X = S.in + a * S.in_stride;
+    u32 *Y = S.out + a * S.out_stride;
     const int64_t n2 = (int64_t)1 << logn2;
     const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
-    constexpr int LOM = (1 << kOuterLoBits) - 1;
+    constexpr int PER = TL::WORDS / TL::T;
+    u32 xa[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
+        xa[r] = X[i * n2 + c0 + c];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
+    __syncthreads();
+    cm_tile_dit<TL>(lds, lvi, p);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
-        const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
-        const uint2 a = oilo[e & LOM], b = oihi[e >> kOuterLoBits];
-        u32 v = cm_shoup_lazy(X[i * n2 + c0 + c], a.x, a.y, p);
-        lds[cm_pad(idx)] = cm_shoup(v, b.x, b.y, p);
+        const int64_t e = i * n2 + c0 + c;
+        if (e < S.out_len) Y[e] = lds[cm_pad(idx)];
+    }
+}
+
+// Row pass of the four-step multiply: row r of F and G (r = a * n1 + j, the
+// row holding frequency k1 = brev(j)); elements i = t + T m are multiplied by
+// the outer twiddle w_n^(k1 i) = rb[k1 T + t] * rc[16 k1 + m] on the way in,
+// the two rows are multiplied in LDS (forward, pointwise, inverse), and the
+// inverse outer twiddle is applied on the way out.  In place on F.
+// Occupancy target: 2 workgroups per CU for 512-thread rows (4 waves/EU),
+// 4 for 256-thread rows.
+template <int LOGL2>
+__global__ __launch_bounds__((Tile<LOGL2, 1, false>::T))
+__attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
+    u32 *F, const u32 *__restrict__ G, int logn1, const uint2 *__restrict__ rb,
+    const uint2 *__restrict__ rc, const uint2 *__restrict__ rbi, const uint2 *__restrict__ rci,
+    const uint2 *__restrict__ lv, const uint2 *__restrict__ lvi, u32 p, u32 pinv, uint2 sc) {
+    using TL = Tile<LOGL2, 1, false>;
+    constexpr int T = TL::T;
+    static_assert(TL::L == 16 * T, "row pass assumes 16 elements per thread");
+    __shared__ u32 lds[2 * TL::LDS_WORDS];
+    u32 *A = lds, *B = lds + TL::LDS_WORDS;
+    const int64_t r = blockIdx.x;
+    const int k1 = (int)cm_brev((u32)(r & ((1 << logn1) - 1)), logn1);
+    u32 *Fr = F + r * TL::L;
+    const u32 *Gr = G + r * TL::L;
+    const int t = threadIdx.x;
+    const uint2 b = rb[(int64_t)k1 * T + t];
+    u32 fa[16], ga[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        fa[m] = Fr[t + T * m];
+        ga[m] = Gr[t + T * m];
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int i = t + T * m;
+        const uint2 c = rc[k1 * 16 + m];
+        A[cm_pad(i)] = cm_shoup(cm_shoup_lazy(fa[m], c.x, c.y, p), b.x, b.y, p);
+        B[cm_pad(i)] = cm_shoup(cm_shoup_lazy(ga[m], c.x, c.y, p), b.x, b.y, p);
     }
     __syncthreads();
-    cm_tile_dit<TL>(lds, twi, tws, p);
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
-        const int i = idx / C, c = idx % C;
-        X[i * n2 + c0 + c] = lds[cm_pad(idx)];
+    cm_tile_multiply<TL>(A, B, lv, lvi, p, pinv, sc);
+    const uint2 bi = rbi[(int64_t)k1 * T + t];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int i = t + T * m;
+        const uint2 c = rci[k1 * 16 + m];
+        Fr[i] = cm_shoup(cm_shoup_lazy(A[cm_pad(i)], c.x, c.y, p), bi.x, bi.y, p);
     }
 }
 
@@ -178,7 +262,7 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_inv(
 template <int LOGL2, int CR>
 __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
     const u32 *__restrict__ X0, u32 *__restrict__ Y0, int64_t stride, int logn1,
-    const uint2 *__restrict__ tw, int tws, u32 p, uint2 sc) {
+    const uint2 *__restrict__ lv, u32 p, uint2 sc) {
     using TL = Tile<LOGL2, CR, false>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const u32 *X = X0 + (int64_t)blockIdx.y * stride;
@@ -186,13 +270,20 @@ __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
     const int L = TL::L;
     const int64_t n1 = (int64_t)1 << logn1;
     const int k10 = blockIdx.x * CR;
+    constexpr int PER = TL::WORDS / TL::T;
+    u32 xa[PER];
 #pragma unroll
-    for (int c = 0; c < CR; ++c) {
-        const int64_t row = cm_brev(k10 + c, logn1);
-        for (int i = threadIdx.x; i < L; i += TL::T) lds[TL::addr(c, i)] = X[row * L + i];
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL2, i = idx & (L - 1);
+        xa[r] = X[(int64_t)cm_brev(k10 + c, logn1) * L + i];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL2, i = idx & (L - 1);
+        lds[TL::addr(c, i)] = xa[r];
     }
     __syncthreads();
-    cm_tile_dif<TL>(lds, tw, tws, p);
+    cm_tile_dif<TL>(lds, lv, p);
     const bool scale = sc.x != 1u;
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int k2 = idx / CR, c = idx % CR;
@@ -207,16 +298,21 @@ __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
 // src/list_stockham.cu:1193-1201); output scaled by sc when sc.x != 1.
 template <int LOGM, int C>
 __global__ __launch_bounds__((Tile<LOGM, C, true>::T)) void k_col_ntt(
-    u32 *__restrict__ X, int64_t ncols, const uint2 *__restrict__ tw, int tws, u32 p, uint2 sc) {
+    u32 *__restrict__ X, int64_t ncols, const uint2 *__restrict__ lv, u32 p, uint2 sc) {
     using TL = Tile<LOGM, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int64_t c0 = (int64_t)blockIdx.x * C;
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
-        const int i = idx / C, c = idx % C;
-        lds[cm_pad(idx)] = (c0 + c < ncols) ? X[i * ncols + c0 + c] : 0u;
+    constexpr int PER = TL::WORDS / TL::T;
+    u32 xa[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
+        xa[r] = (c0 + c < ncols) ? X[i * ncols + c0 + c] : 0u;
     }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
-    cm_tile_dif<TL>(lds, tw, tws, p);
+    cm_tile_dif<TL>(lds, lv, p);
     const bool scale = sc.x != 1u;
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
@@ -287,12 +383,10 @@ static hipError_t mul_lds_t(const u32 *F, const u32 *G, u32 *H, const MulDesc *d
                             const MulDesc &one, const Family *fam, hipStream_t s) {
     constexpr int C = rows_per_wg<LOGL>();
     using TL = Tile<LOGL, C, false>;
-    const int tws = fam->LT - LOGL;
     const int nb = (ndesc + C - 1) / C;
     ProfScope ps(kProfMulLds, s);
     hipLaunchKernelGGL((k_mul_lds<LOGL, C, DESC>), dim3(nb), dim3(TL::T), 0, s, F, G, H, descs,
-                       ndesc, one, fam->fwd, fam->inv, tws, fam->p, fam->pinv,
-                       mul_scale(LOGL, fam->p));
+                       ndesc, one, fam->lv, fam->lvi, fam->p, fam->pinv, mul_scale(LOGL, fam->p));
     return hipGetLastError();
 }
 
@@ -316,22 +410,23 @@ hipError_t launch_mul_lds_one(int logn, const u32 *F, const u32 *G, u32 *H,
 }
 
 template <int LOGL>
-static hipError_t ntt_lds_t(u32 *X, int64_t m, const uint2 *tw, uint2 sc, const Family *fam,
+static hipError_t ntt_lds_t(u32 *X, int64_t m, const uint2 *lv, uint2 sc, const Family *fam,
                             hipStream_t s) {
     constexpr int C = rows_per_wg<LOGL>();
     using TL = Tile<LOGL, C, false>;
     const int64_t nb = (m + C - 1) / C;
     ProfScope ps(kProfNttLds, s);
-    hipLaunchKernelGGL((k_ntt_lds<LOGL, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, m, tw,
-                       fam->LT - LOGL, fam->p, sc);
+    hipLaunchKernelGGL((k_ntt_lds<LOGL, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, m, lv,
+                       fam->p, sc);
     return hipGetLastError();
 }
 
-hipError_t launch_ntt_lds(int k, u32 *X, int64_t m, const uint2 *tw, uint2 sc, const Family *fam,
+hipError_t launch_ntt_lds(int k, u32 *X, int64_t m, bool inverse, uint2 sc, const Family *fam,
                           hipStream_t s) {
     if (m <= 0) return hipSuccess;
+    const uint2 *lv = inverse ? fam->lvi : fam->lv;
     switch (k) {
-#define CM_CASE(K) case K: return ntt_lds_t<K>(X, m, tw, sc, fam, s);
+#define CM_CASE(K) case K: return ntt_lds_t<K>(X, m, lv, sc, fam, s);
         CM_CASE(4) CM_CASE(5) CM_CASE(6) CM_CASE(7) CM_CASE(8) CM_CASE(9) CM_CASE(10)
         CM_CASE(11) CM_CASE(12) CM_CASE(13)
 #undef CM_CASE
@@ -346,39 +441,43 @@ static constexpr int col_tile() {
 }
 
 template <int LOGL1>
-static hipError_t col_fwd_t(u32 *X0, int cnt0, u32 *X1, int cnt1, int64_t stride, int logn2,
-                            const Family *fam, const Outer *o, hipStream_t s) {
+static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
+                            const uint2 *lv, const uint2 *olo, const uint2 *ohi, u32 p,
+                            hipStream_t s) {
     constexpr int C = col_tile<LOGL1>();
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColFwd, s);
-    hipLaunchKernelGGL((k_col_fwd<LOGL1, C>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0, s, X0,
-                       cnt0, X1, stride, logn2, fam->fwd, fam->LT - LOGL1, o->lo_f, o->hi_f,
-                       fam->p);
+    if (olo)
+        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0, s,
+                           s0, s1, cnt0, logn2, lv, olo, ohi, p);
+    else
+        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0,
+                           s, s0, s1, cnt0, logn2, lv, olo, ohi, p);
     return hipGetLastError();
 }
 
 template <int LOGL1>
-static hipError_t col_inv_t(u32 *X, int cnt, int64_t stride, int logn2, const Family *fam,
-                            const Outer *o, hipStream_t s) {
+static hipError_t col_inv_t(const ColSet &S, int cnt, int logn2, const Family *fam,
+                            hipStream_t s) {
     constexpr int C = col_tile<LOGL1>();
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColInv, s);
-    hipLaunchKernelGGL((k_col_inv<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, X, stride,
-                       logn2, fam->inv, fam->LT - LOGL1, o->lo_i, o->hi_i, fam->p);
+    hipLaunchKernelGGL((k_col_inv<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S, logn2,
+                       fam->lvi, fam->p);
     return hipGetLastError();
 }
 
 template <int LOGL2>
 static hipError_t row_out_t(const u32 *X, u32 *Y, int cnt, int64_t stride, int logn1,
-                            const uint2 *tw, uint2 sc, const Family *fam, hipStream_t s) {
-    constexpr int CR = LOGL2 >= 12 ? (1 << (14 - LOGL2)) : 4;
+                            const uint2 *lv, uint2 sc, const Family *fam, hipStream_t s) {
+    constexpr int CR = LOGL2 >= 12 ? (1 << (14 - LOGL2)) : 4;   // 16384 words per tile
     using TL = Tile<LOGL2, CR, false>;
     const int nb = (int)(((int64_t)1 << logn1) / CR);
     ProfScope ps(kProfRowOut, s);
     hipLaunchKernelGGL((k_row_out<LOGL2, CR>), dim3(nb, cnt), dim3(TL::T), 0, s, X, Y, stride,
-                       logn1, tw, fam->LT - LOGL2, fam->p, sc);
+                       logn1, lv, fam->p, sc);
     return hipGetLastError();
 }
 
@@ -394,47 +493,46 @@ static hipError_t row_out_t(const u32 *X, u32 *Y, int cnt, int64_t stride, int l
     }                                                                                    \
     return hipErrorInvalidValue;
 
-static hipError_t col_fwd(int k1, u32 *X0, int cnt0, u32 *X1, int cnt1, int64_t stride,
-                          int logn2, const Family *fam, const Outer *o, hipStream_t s) {
-    CM_K1_CASES(col_fwd_t, X0, cnt0, X1, cnt1, stride, logn2, fam, o, s)
+static hipError_t col_fwd(int k1, const ColSet &s0, const ColSet &s1, int cnt0, int cnt1,
+                          int logn2, const uint2 *lv, const uint2 *olo, const uint2 *ohi, u32 p,
+                          hipStream_t s) {
+    CM_K1_CASES(col_fwd_t, s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, p, s)
 }
 
-static hipError_t col_inv(int k1, u32 *X, int cnt, int64_t stride, int logn2, const Family *fam,
-                          const Outer *o, hipStream_t s) {
-    CM_K1_CASES(col_inv_t, X, cnt, stride, logn2, fam, o, s)
+static hipError_t col_inv(int k1, const ColSet &S, int cnt, int logn2, const Family *fam,
+                          hipStream_t s) {
+    CM_K1_CASES(col_inv_t, S, cnt, logn2, fam, s)
 }
 
-hipError_t launch_mul_four_step(int k, u32 *F, u32 *G, int cnt, const Family *fam,
-                                hipStream_t s) {
+hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, hipStream_t s) {
     if (k <= kMaxLdsLog || k > kMaxLog || k > fam->K) return hipErrorInvalidValue;
     int k1, k2;
     four_step_split(k, k1, k2);
     const Outer *o = outer_for(fam, k);
     if (!o) return hipErrorOutOfMemory;
     const int64_t n = (int64_t)1 << k;
-    hipError_t e = col_fwd(k1, F, cnt, G, cnt, n, k2, fam, o, s);
+    ColSet sf{io.f_in, io.f_ws, io.in_stride, io.f_len, n, n};
+    ColSet sg{io.g_in, io.g_ws, io.in_stride, io.g_len, n, n};
+    hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr, fam->p, s);
     if (e != hipSuccess) return e;
-    // row multiply: one row of F and G per workgroup (2^k1 * cnt rows)
-    const int nrows = (int)(((int64_t)cnt << k1));
-    uint2 sc = mul_scale(k, fam->p);
-    const int tws = fam->LT - k2;
+    // row pass: one row of F and G per workgroup (2^k1 * cnt rows)
+    const int nrows = (int)(((int64_t)io.cnt << k1));
+    const uint2 sc = mul_scale(k, fam->p);
     {
-    ProfScope ps(kProfRowMul, s);
-    if (k2 == 12) {
-        using TL = Tile<12, 1, false>;
-        hipLaunchKernelGGL((k_mul_lds<12, 1, false>), dim3(nrows), dim3(TL::T), 0, s, F, G, F,
-                           nullptr, nrows, MulDesc{}, fam->fwd, fam->inv, tws, fam->p,
-                           fam->pinv, sc);
-    } else {
-        using TL = Tile<13, 1, false>;
-        hipLaunchKernelGGL((k_mul_lds<13, 1, false>), dim3(nrows), dim3(TL::T), 0, s, F, G, F,
-                           nullptr, nrows, MulDesc{}, fam->fwd, fam->inv, tws, fam->p,
-                           fam->pinv, sc);
-    }
+        ProfScope ps(kProfRowMul, s);
+#define CM_ROW(K2)                                                                         \
+    hipLaunchKernelGGL((k_row_mul<K2>), dim3(nrows), dim3(Tile<K2, 1, false>::T), 0, s,    \
+                       io.f_ws, io.g_ws, k1, o->rb_f, o->rc_f, o->rb_i, o->rc_i, fam->lv,  \
+                       fam->lvi, fam->p, fam->pinv, sc)
+        if (k2 == 12) CM_ROW(12);
+        else if (k2 == 13) CM_ROW(13);
+        else CM_ROW(14);
+#undef CM_ROW
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return col_inv(k1, F, cnt, n, k2, fam, o, s);
+    ColSet sh{io.f_ws, io.h_out, n, n, io.out_stride, io.h_len};
+    return col_inv(k1, sh, io.cnt, k2, fam, s);
 }
 
 hipError_t launch_ntt_four_step(int k, u32 *X, u32 *Y, int cnt, bool inverse, const Family *fam,
@@ -446,40 +544,35 @@ hipError_t launch_ntt_four_step(int k, u32 *X, u32 *Y, int cnt, bool inverse, co
     if (!o) return hipErrorOutOfMemory;
     const int64_t n = (int64_t)1 << k;
     // the inverse transform is the forward transform with the inverse root
-    Outer oi = *o;
-    const uint2 *tw = fam->fwd;
-    if (inverse) {
-        oi.lo_f = o->lo_i; oi.hi_f = o->hi_i;
-        tw = fam->inv;
-    }
-    Family fi = *fam;
-    fi.fwd = const_cast<uint2 *>(tw);
-    hipError_t e = col_fwd(k1, X, cnt, X, 0, n, k2, &fi, &oi, s);
+    const uint2 *lv = inverse ? fam->lvi : fam->lv;
+    const uint2 *olo = inverse ? o->lo_i : o->lo_f, *ohi = inverse ? o->hi_i : o->hi_f;
+    ColSet sx{X, X, n, n, n, n};
+    hipError_t e = col_fwd(k1, sx, sx, cnt, 0, k2, lv, olo, ohi, fam->p, s);
     if (e != hipSuccess) return e;
     uint2 sc = make_uint2(1u, 0u);
     if (inverse) sc = shoup_pair(cm_host_inv(cm_host_pow(2, k, fam->p), fam->p), fam->p);
-    if (k2 == 12) e = row_out_t<12>(X, Y, cnt, n, k1, tw, sc, fam, s);
-    else e = row_out_t<13>(X, Y, cnt, n, k1, tw, sc, fam, s);
+    if (k2 == 12) e = row_out_t<12>(X, Y, cnt, n, k1, lv, sc, fam, s);
+    else if (k2 == 13) e = row_out_t<13>(X, Y, cnt, n, k1, lv, sc, fam, s);
+    else e = row_out_t<14>(X, Y, cnt, n, k1, lv, sc, fam, s);
     if (e != hipSuccess) return e;
     return hipMemcpyAsync(X, Y, sizeof(u32) * (size_t)(n * cnt), hipMemcpyDeviceToDevice, s);
 }
 
 template <int LOGM>
-static hipError_t col_ntt_t(u32 *X, int64_t ncols, const uint2 *tw, uint2 sc, const Family *fam,
-                            hipStream_t s) {
+static hipError_t col_ntt_t(u32 *X, int64_t ncols, uint2 sc, const Family *fam, hipStream_t s) {
     constexpr int C = col_tile<LOGM>();
     using TL = Tile<LOGM, C, true>;
     const int64_t nb = (ncols + C - 1) / C;
     ProfScope ps(kProfColNtt, s);
-    hipLaunchKernelGGL((k_col_ntt<LOGM, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, ncols, tw,
-                       fam->LT - LOGM, fam->p, sc);
+    hipLaunchKernelGGL((k_col_ntt<LOGM, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, ncols,
+                       fam->lv, fam->p, sc);
     return hipGetLastError();
 }
 
-hipError_t launch_col_ntt(int em, u32 *X, int64_t ncols, const uint2 *tw, uint2 sc,
-                          const Family *fam, hipStream_t s) {
+hipError_t launch_col_ntt(int em, u32 *X, int64_t ncols, uint2 sc, const Family *fam,
+                          hipStream_t s) {
     const int k1 = em;
-    CM_K1_CASES(col_ntt_t, X, ncols, tw, sc, fam, s)
+    CM_K1_CASES(col_ntt_t, X, ncols, sc, fam, s)
 }
 
 hipError_t launch_pointwise(u32 *Y, const u32 *X, int64_t n, u32 p, hipStream_t s) {

@@ -6,6 +6,8 @@
 // one round applies R consecutive radix-2 stages to groups of 2^R elements
 // that each thread holds in registers, so a 4096-point transform is three
 // register-resident radix-16 passes with two LDS exchanges in between.
+// Rounds can run on NV tiles at once (the two operands of a product), so
+// each twiddle is loaded once for both.
 //
 // Forward transforms are decimation in frequency (natural order in,
 // bit-reversed order out); inverse transforms are the exact stage-by-stage
@@ -17,10 +19,13 @@
 // the tile back through a bit reversal.
 //
 // DIF stage s (0-based) pairs positions q and q + d, d = L >> (s+1), inside
-// blocks of 2d words, and maps (x, y) -> (x + y, (x - y) * w_L^((q mod d) 2^s)).
+// blocks of 2d words, and maps (x, y) -> (x + y, (x - y) * w_{2d}^(q mod d)).
 //
-// Twiddle tables hold uint2 {w^j, floor(w^j 2^32 / p)} for a root of order
-// Ltab >= L; entry j << tws of the table is w_L^j with tws = log2(Ltab/L).
+// Twiddles come from a level table: for every M = 2, 4, ..., 2^LT the M/2
+// entries w_M^0 .. w_M^(M/2-1) (uint2 {w, floor(w 2^32 / p)}) are stored
+// contiguously at offset M/2 - 1.  Stage s of an L-point transform reads
+// level M = L >> s at index q mod d, so consecutive lanes read consecutive
+// entries (one or a few cache lines per wave instruction).
 #pragma once
 
 #include "modarith.h"
@@ -86,43 +91,48 @@ struct Tile {
     }
 };
 
-// R DIF stages on a register group (stages S0 .. S0+R-1).
-template <int R, int S0, int DLOG>
-CM_DEV void cm_dif_stages(u32 (&v)[1 << R], int gi_lo, const uint2 *__restrict__ tw, int tws,
-                          u32 p) {
+// R DIF stages (S0 .. S0+R-1 of an L = 2^LOGL transform) on NV register
+// groups that share their positions (and so their twiddles).
+template <int LOGL, int R, int S0, int DLOG, int NV>
+CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restrict__ lv, u32 p) {
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         const int h = 1 << (R - 1 - t);
+        const int base = (1 << (LOGL - S0 - t - 1)) - 1;     // level L >> (S0+t)
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) {
             if (m & h) continue;
             const int mm = (m & (h - 1)) << DLOG;
             if (DLOG == 0 && mm == 0) {
-                cm_dif_bfly1(v[m], v[m + h], p);
+#pragma unroll
+                for (int a = 0; a < NV; ++a) cm_dif_bfly1(v[a][m], v[a][m + h], p);
             } else {
-                const int e = ((DLOG == 0 ? 0 : gi_lo) + mm) << (S0 + t);
-                cm_dif_bfly(v[m], v[m + h], tw[e << tws], p);
+                const uint2 w = lv[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+#pragma unroll
+                for (int a = 0; a < NV; ++a) cm_dif_bfly(v[a][m], v[a][m + h], w, p);
             }
         }
     }
 }
 
-// Inverse of cm_dif_stages with the inverse twiddle table.
-template <int R, int S0, int DLOG>
-CM_DEV void cm_dit_stages(u32 (&v)[1 << R], int gi_lo, const uint2 *__restrict__ twi, int tws,
-                          u32 p) {
+// Inverse of cm_dif_stages with the inverse level table.
+template <int LOGL, int R, int S0, int DLOG, int NV>
+CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restrict__ lvi, u32 p) {
 #pragma unroll
     for (int t = R - 1; t >= 0; --t) {
         const int h = 1 << (R - 1 - t);
+        const int base = (1 << (LOGL - S0 - t - 1)) - 1;
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) {
             if (m & h) continue;
             const int mm = (m & (h - 1)) << DLOG;
             if (DLOG == 0 && mm == 0) {
-                cm_dit_bfly1(v[m], v[m + h], p);
+#pragma unroll
+                for (int a = 0; a < NV; ++a) cm_dit_bfly1(v[a][m], v[a][m + h], p);
             } else {
-                const int e = ((DLOG == 0 ? 0 : gi_lo) + mm) << (S0 + t);
-                cm_dit_bfly(v[m], v[m + h], twi[e << tws], p);
+                const uint2 w = lvi[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+#pragma unroll
+                for (int a = 0; a < NV; ++a) cm_dit_bfly(v[a][m], v[a][m + h], w, p);
             }
         }
     }
@@ -148,104 +158,107 @@ struct Round {
         CM_DEV int a(int m) const { return TL::addr(c, i0 + (m << DLOG)); }
     };
 
-    static CM_DEV void load(const u32 *lds, const G &g, u32 (&v)[1 << R]) {
+    template <int NV>
+    static CM_DEV void load(u32 *const (&lds)[NV], const G &g, u32 (&v)[NV][1 << R]) {
 #pragma unroll
-        for (int m = 0; m < (1 << R); ++m) v[m] = lds[g.a(m)];
-    }
-    static CM_DEV void store(u32 *lds, const G &g, const u32 (&v)[1 << R]) {
+        for (int a = 0; a < NV; ++a)
 #pragma unroll
-        for (int m = 0; m < (1 << R); ++m) lds[g.a(m)] = v[m];
+            for (int m = 0; m < (1 << R); ++m) v[a][m] = lds[a][g.a(m)];
     }
-    // full forward round, LDS -> LDS
-    static CM_DEV void dif(u32 *lds, const uint2 *tw, int tws, u32 p) {
+    template <int NV>
+    static CM_DEV void store(u32 *const (&lds)[NV], const G &g, const u32 (&v)[NV][1 << R]) {
+#pragma unroll
+        for (int a = 0; a < NV; ++a)
+#pragma unroll
+            for (int m = 0; m < (1 << R); ++m) lds[a][g.a(m)] = v[a][m];
+    }
+    // full forward round on NV tiles, LDS -> LDS
+    template <int NV>
+    static CM_DEV void dif(u32 *const (&lds)[NV], const uint2 *lv, u32 p) {
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             G g(threadIdx.x + q * TL::T);
-            u32 v[1 << R];
-            load(lds, g, v);
-            cm_dif_stages<R, S0, DLOG>(v, g.gi_lo, tw, tws, p);
-            store(lds, g, v);
+            u32 v[NV][1 << R];
+            load<NV>(lds, g, v);
+            cm_dif_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lv, p);
+            store<NV>(lds, g, v);
         }
     }
-    // full inverse round, LDS -> LDS
-    static CM_DEV void dit(u32 *lds, const uint2 *twi, int tws, u32 p) {
+    // full inverse round on NV tiles, LDS -> LDS
+    template <int NV>
+    static CM_DEV void dit(u32 *const (&lds)[NV], const uint2 *lvi, u32 p) {
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             G g(threadIdx.x + q * TL::T);
-            u32 v[1 << R];
-            load(lds, g, v);
-            cm_dit_stages<R, S0, DLOG>(v, g.gi_lo, twi, tws, p);
-            store(lds, g, v);
+            u32 v[NV][1 << R];
+            load<NV>(lds, g, v);
+            cm_dit_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lvi, p);
+            store<NV>(lds, g, v);
         }
     }
 };
 
-// Forward rounds [J, J1) with a barrier after each.
-template <class TL, int J, int J1>
-CM_DEV void cm_dif_rounds(u32 *lds, const uint2 *tw, int tws, u32 p) {
+// Forward rounds [J, J1) on NV tiles with a barrier after each.
+template <class TL, int J, int J1, int NV>
+CM_DEV void cm_dif_rounds(u32 *const (&lds)[NV], const uint2 *lv, u32 p) {
     if constexpr (J < J1) {
-        Round<TL, J>::dif(lds, tw, tws, p);
+        Round<TL, J>::template dif<NV>(lds, lv, p);
         __syncthreads();
-        cm_dif_rounds<TL, J + 1, J1>(lds, tw, tws, p);
+        cm_dif_rounds<TL, J + 1, J1, NV>(lds, lv, p);
     }
 }
 
 // Inverse rounds J, J-1, ..., J0 (J >= J0) with a barrier after each.
-template <class TL, int J, int J0>
-CM_DEV void cm_dit_rounds(u32 *lds, const uint2 *twi, int tws, u32 p) {
+template <class TL, int J, int J0, int NV>
+CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const uint2 *lvi, u32 p) {
     if constexpr (J >= J0) {
-        Round<TL, J>::dit(lds, twi, tws, p);
+        Round<TL, J>::template dit<NV>(lds, lvi, p);
         __syncthreads();
-        cm_dit_rounds<TL, J - 1, J0>(lds, twi, tws, p);
+        cm_dit_rounds<TL, J - 1, J0, NV>(lds, lvi, p);
     }
 }
 
-// Full forward transform of every vector in the tile (leaves bit-reversed
-// order, barrier included).
+// Full forward transform of every vector of one tile (bit-reversed result).
 template <class TL>
-CM_DEV void cm_tile_dif(u32 *lds, const uint2 *tw, int tws, u32 p) {
-    cm_dif_rounds<TL, 0, TL::NR>(lds, tw, tws, p);
+CM_DEV void cm_tile_dif(u32 *lds, const uint2 *lv, u32 p) {
+    u32 *const t[1] = {lds};
+    cm_dif_rounds<TL, 0, TL::NR, 1>(t, lv, p);
 }
 
 template <class TL>
-CM_DEV void cm_tile_dit(u32 *lds, const uint2 *twi, int tws, u32 p) {
-    cm_dit_rounds<TL, TL::NR - 1, 0>(lds, twi, tws, p);
+CM_DEV void cm_tile_dit(u32 *lds, const uint2 *lvi, u32 p) {
+    u32 *const t[1] = {lds};
+    cm_dit_rounds<TL, TL::NR - 1, 0, 1>(t, lvi, p);
 }
 
-// Pointwise product of two transformed tiles held as the last forward round
-// (fr: first operand kept in registers, lds: second operand), scaled by the
-// Shoup constant sc (= 2^32 / N mod p, cancelling Montgomery's 2^-32 and
-// the inverse transform's factor N), followed by the first inverse round;
-// the result is written back to LDS.  Barrier included.
+// Multiply of two tiles A (first operand, becomes the product) and B:
+// forward transforms of both (shared twiddles), pointwise product scaled by
+// the Shoup constant sc (= 2^32 / N mod p, cancelling Montgomery's 2^-32 and
+// the inverse transform's factor N) fused with the last forward and first
+// inverse round, then the remaining inverse rounds on A.  Barriers included;
+// the product is in A afterwards.
 template <class TL>
-struct LastRound {
+CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const uint2 *lv, const uint2 *lvi, u32 p, u32 pinv,
+                             uint2 sc) {
+    u32 *const ab[2] = {A, B};
+    cm_dif_rounds<TL, 0, TL::NR - 1, 2>(ab, lv, p);
     using RD = Round<TL, TL::NR - 1>;
-    static constexpr int R = RD::R;
-    static constexpr int GPT = RD::GPT;
-    // forward last round of the first operand, kept in registers
-    static CM_DEV void dif_keep(const u32 *lds, u32 (&fr)[GPT][1 << R], const uint2 *tw, int tws,
-                                u32 p) {
+    constexpr int R = RD::R;
 #pragma unroll
-        for (int q = 0; q < GPT; ++q) {
-            typename RD::G g(threadIdx.x + q * TL::T);
-            RD::load(lds, g, fr[q]);
-            cm_dif_stages<R, RD::S0, RD::DLOG>(fr[q], g.gi_lo, tw, tws, p);
-        }
+    for (int q = 0; q < RD::GPT; ++q) {
+        typename RD::G g(threadIdx.x + q * TL::T);
+        u32 v[2][1 << R];
+        RD::template load<2>(ab, g, v);
+        cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 2>(v, g.gi_lo, lv, p);
+        u32 w[1][1 << R];
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m)
+            w[0][m] = cm_shoup(cm_mont(v[0][m], v[1][m], p, pinv), sc.x, sc.y, p);
+        cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(w, g.gi_lo, lvi, p);
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) A[g.a(m)] = w[0][m];
     }
-    static CM_DEV void dif_mul_dit(u32 *lds, const u32 (&fr)[GPT][1 << R], const uint2 *tw,
-                                   const uint2 *twi, int tws, u32 p, u32 pinv, uint2 sc) {
-#pragma unroll
-        for (int q = 0; q < GPT; ++q) {
-            typename RD::G g(threadIdx.x + q * TL::T);
-            u32 v[1 << R];
-            RD::load(lds, g, v);
-            cm_dif_stages<R, RD::S0, RD::DLOG>(v, g.gi_lo, tw, tws, p);
-#pragma unroll
-            for (int m = 0; m < (1 << R); ++m)
-                v[m] = cm_shoup(cm_mont(fr[q][m], v[m], p, pinv), sc.x, sc.y, p);
-            cm_dit_stages<R, RD::S0, RD::DLOG>(v, g.gi_lo, twi, tws, p);
-            RD::store(lds, g, v);
-        }
-        __syncthreads();
-    }
-};
+    __syncthreads();
+    u32 *const a1[1] = {A};
+    cm_dit_rounds<TL, TL::NR - 2, 0, 1>(a1, lvi, p);
+}

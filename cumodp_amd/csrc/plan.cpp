@@ -5,6 +5,8 @@
 // doubling launches) and cudaMallocs every temporary.  Here each root family
 // is built once per process, on the host, with Shoup companions, and kept in
 // device memory; scratch space grows and is reused.
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -22,7 +24,8 @@ namespace {
 
 std::mutex g_plan_mutex;
 std::map<std::tuple<u32, u32, int>, Family *> g_families;
-std::map<std::pair<const Family *, int>, Outer *> g_outer;
+std::map<std::tuple<const Family *, int, int>, Outer *> g_outer;
+int g_split[kMaxLog + 1];   // chosen k2 per k (0: default)
 std::map<u32, const Family *> g_default;
 
 uint2 *upload(const std::vector<uint2> &h) {
@@ -52,12 +55,20 @@ Family *make_family(u32 p, u32 wN, int K) {
     f->pinv = cm_host_pinv(p);
     f->K = K;
     f->wN = wN;
-    f->LT = K < kMaxLdsLog ? K : kMaxLdsLog;
-    u32 wt = cm_host_pow(wN, (u64)1 << (K - f->LT), p);   // order 2^LT
-    size_t half = f->LT >= 1 ? ((size_t)1 << (f->LT - 1)) : 1;
-    f->fwd = upload(power_table(wt, half, p));
-    f->inv = upload(power_table(cm_host_inv(wt, p), half, p));
-    if (!f->fwd || !f->inv) {
+    f->LT = K < kMaxRowLog ? K : kMaxRowLog;
+    // level tables: level M = 2^j (j = 1 .. LT) at offset 2^(j-1) - 1
+    std::vector<uint2> lv, lvi;
+    for (int j = 1; j <= f->LT; ++j) {
+        u32 wm = cm_host_pow(wN, (u64)1 << (K - j), p);   // order 2^j
+        std::vector<uint2> a = power_table(wm, (size_t)1 << (j - 1), p);
+        std::vector<uint2> b = power_table(cm_host_inv(wm, p), (size_t)1 << (j - 1), p);
+        lv.insert(lv.end(), a.begin(), a.end());
+        lvi.insert(lvi.end(), b.begin(), b.end());
+    }
+    if (lv.empty()) { lv.push_back(make_uint2(1, 0)); lvi.push_back(make_uint2(1, 0)); }
+    f->lv = upload(lv);
+    f->lvi = upload(lvi);
+    if (!f->lv || !f->lvi) {
         delete f;
         return nullptr;
     }
@@ -94,9 +105,35 @@ const Family *family_default(u32 p) {
     return f;
 }
 
-const Outer *outer_for(const Family *f, int k) {
+void four_step_split(int k, int &k1, int &k2) {
+    int c = (k >= 0 && k <= kMaxLog) ? g_split[k] : 0;
+    if (!c) {
+        char name[32];
+        snprintf(name, sizeof(name), "CUMODP_SPLIT_%d", k);
+        const char *e = getenv(name);
+        if (e) c = atoi(e);
+    }
+    // measured default (2^24 on MI355X: 13 beats 12 and 14): wider column
+    // tiles once the columns are long enough
+    if (c < 12 || c > 14 || k - c < 1 || k - c > kMaxLdsLog) c = k >= 21 ? 13 : 12;
+    k2 = c;
+    k1 = k - k2;
+}
+
+bool set_four_step_split(int k, int k2) {
+    if (k <= kMaxLdsLog || k > kMaxLog || k2 < 12 || k2 > 14 || k - k2 < 1 ||
+        k - k2 > kMaxLdsLog)
+        return false;
     std::lock_guard<std::mutex> lk(g_plan_mutex);
-    auto key = std::make_pair(f, k);
+    g_split[k] = k2;
+    return true;
+}
+
+const Outer *outer_for(const Family *f, int k) {
+    int k1, k2;
+    four_step_split(k, k1, k2);
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto key = std::make_tuple(f, k, k2);
     auto it = g_outer.find(key);
     if (it != g_outer.end()) return it->second;
     if (k > f->K || k <= kOuterLoBits) return nullptr;
@@ -110,7 +147,30 @@ const Outer *outer_for(const Family *f, int k) {
     o->hi_f = upload(power_table(cm_host_pow(wn, nlo, p), nhi, p));
     o->lo_i = upload(power_table(wni, nlo, p));
     o->hi_i = upload(power_table(cm_host_pow(wni, nlo, p), nhi, p));
-    if (!o->lo_f || !o->hi_f || !o->lo_i || !o->hi_i) {
+    // row-pass tables
+    const size_t n1 = (size_t)1 << k1, T = ((size_t)1 << k2) / 16;
+    for (int dir = 0; dir < 2; ++dir) {
+        const u32 w = dir ? wni : wn;
+        std::vector<uint2> rb(n1 * T), rc(n1 * 16);
+        for (size_t r = 0; r < n1; ++r) {
+            const u32 wr = cm_host_pow(w, r, p);                 // w^k1
+            u64 x = 1;
+            for (size_t t = 0; t < T; ++t) {
+                rb[r * T + t] = make_uint2((u32)x, cm_host_shoup_q((u32)x, p));
+                x = x * wr % p;
+            }
+            const u32 wrT = cm_host_pow(wr, T, p);              // w^(k1 T)
+            x = 1;
+            for (size_t m = 0; m < 16; ++m) {
+                rc[r * 16 + m] = make_uint2((u32)x, cm_host_shoup_q((u32)x, p));
+                x = x * wrT % p;
+            }
+        }
+        (dir ? o->rb_i : o->rb_f) = upload(rb);
+        (dir ? o->rc_i : o->rc_f) = upload(rc);
+    }
+    if (!o->lo_f || !o->hi_f || !o->lo_i || !o->hi_i || !o->rb_f || !o->rc_f || !o->rb_i ||
+        !o->rc_i) {
         delete o;
         return nullptr;
     }

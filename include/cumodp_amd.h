@@ -54,6 +54,12 @@ typedef struct {
 CUMODP_API cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
                                   const sfixn *G, sfixn *H, sfixn p, void *stream);
 
+/* ---- tuning: four-step split ----------------------------------------------
+ * Transforms of 2^k points above 2^13 run as 2^(k-k2) columns x 2^k2 rows;
+ * this chooses k2 (12, 13 or 14) for one k.  The tuner (cumodp_amd/autotune.py)
+ * measures the candidates and records its choice. */
+CUMODP_API cumodp_err cumodp_tune_set_split(int k, int k2);
+
 /* ---- optional per-kernel timing ------------------------------------------
  * Off unless enabled here or with CUMODP_PROFILE=1.  When on, each hot kernel
  * launch is bracketed by HIP events on its stream; cumodp_profile_read

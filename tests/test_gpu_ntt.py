@@ -95,6 +95,24 @@ def test_host_wrapper_semantics(cm, orc):
     assert (H == -7).all()
 
 
+@pytest.mark.parametrize("k2", [12, 13, 14])
+def test_four_step_splits(cm, orc, k2):
+    """Every supported column/row split of the four-step multiply is exact."""
+    rng = np.random.default_rng(k2)
+    p = P2
+    try:
+        for logn in (15, 19, 22):
+            if logn - k2 > 13 or logn - k2 < 1:
+                continue
+            assert cm.lib().cumodp_tune_set_split(logn, k2) == 0
+            lf, lg = (1 << logn) // 2, (1 << logn) // 2 - 3
+            F, G = rand_poly(rng, lf, p), rand_poly(rng, lg, p)
+            assert (cm.poly_mul(F, G, p) == orc.ntt_polymul(F, G, p)).all(), (logn, k2)
+    finally:
+        for logn in (15, 19, 22):
+            cm.lib().cumodp_tune_set_split(logn, 13 if logn >= 21 else 12)
+
+
 def test_workload2_full_size(cm, orc):
     """The 2^24-point multiply mod 2013265921, bit-exact against the oracle."""
     w = cm.workload2()
