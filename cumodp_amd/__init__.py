@@ -129,6 +129,20 @@ def lib():
         L.cumodp_profile_read.restype = err
         L.cumodp_tune_set_split.argtypes = [s, s]
         L.cumodp_tune_set_split.restype = err
+        i64 = C.c_int64
+        L.cumodp_two_convolution_poly_mul.argtypes = [vp, i64, s, vp, i64, s, vp, s]
+        L.cumodp_two_convolution_poly_mul.restype = err
+        L.cumodp_two_convolution_poly_mul_dev.argtypes = [vp, i64, s, vp, i64, s, vp, s, s, vp]
+        L.cumodp_two_convolution_poly_mul_dev.restype = err
+        L.cumodp_find_largest_bit_width.argtypes = [vp, i64, s, vp, i64, s]
+        L.cumodp_find_largest_bit_width.restype = s
+        L.cumodp_convert_to_modular_bivariate.argtypes = [vp, i64, s, s, s, s, vp]
+        L.cumodp_convert_to_modular_bivariate.restype = err
+        L.cumodp_scale_x_argument_dev.argtypes = [vp, i64, s, s, s, vp]
+        L.cumodp_scale_x_argument_dev.restype = err
+        L.cumodp_two_convolution_plan.argtypes = [s, i64, i64, C.POINTER(s), C.POINTER(s),
+                                                  C.POINTER(s)]
+        L.cumodp_two_convolution_plan.restype = s
         _LIB = L
     return _LIB
 
@@ -279,3 +293,31 @@ def profile_read():
         if c.value:
             out[L.cumodp_profile_name(i).decode()] = (t.value, c.value)
     return out
+
+
+# ---- exact integer multiplication (two convolutions + CRT) -------------------
+def _words(a, W=None):
+    """int64 array or (n, W) uint64 words -> contiguous (n, W) uint64."""
+    a = np.asarray(a)
+    if a.ndim == 1:
+        return np.ascontiguousarray(a.astype(np.int64).view(np.uint64).reshape(-1, 1))
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def two_convolution_poly_mul(a, b, wc):
+    """Exact product of integer polynomials (int64 arrays or (n, W) uint64
+    two's-complement words); returns (na+nb-1, wc) uint64 words."""
+    A, B = _words(a), _words(b)
+    na, wa = A.shape
+    nb, wb = B.shape
+    out = np.zeros((na + nb - 1, wc), np.uint64)
+    _check("cumodp_two_convolution_poly_mul", lib().cumodp_two_convolution_poly_mul(
+        _ptr(A), na, wa, _ptr(B), nb, wb, _ptr(out), wc))
+    return out
+
+
+def two_convolution_plan(bits, na, nb):
+    npr, K, M = C.c_int(), C.c_int(), C.c_int()
+    if not lib().cumodp_two_convolution_plan(bits, na, nb, C.byref(npr), C.byref(K), C.byref(M)):
+        return None
+    return npr.value, K.value, M.value

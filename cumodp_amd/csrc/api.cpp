@@ -1,6 +1,7 @@
 // api.cpp -- the C ABI (include/cumodp.h, include/cumodp_amd.h) and the
 // C++-linkage drop-ins (include/stockham.h, list_stockham.h, fft_aux.h,
 // list_inv_stockham.h) over the gfx950 kernels.
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -355,6 +356,119 @@ cumodp_err cumodp_fftmul_uni_dev(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd,
     const int64_t nout = (int64_t)dh + 1 < n ? (int64_t)dh + 1 : n;
     return mul_dev_impl((u32 *)Hd, nout, (const u32 *)Fd, df + 1, (const u32 *)Gd, dg + 1,
                         (u32)p, as_stream(stream));
+}
+
+// ---- two-convolution integer multiply -------------------------------------
+static int tc_out_bits(int bits_a, int bits_b, int64_t na, int64_t nb) {
+    int lg = 0;
+    while (((int64_t)1 << lg) < std::min(na, nb)) ++lg;
+    return bits_a + bits_b + lg + 1;
+}
+
+// caller holds api_mutex
+static cumodp_err tc_dev_impl(const uint64_t *a, int64_t na, int wa, const uint64_t *b,
+                              int64_t nb, int wb, uint64_t *c, int wc, int max_bits,
+                              hipStream_t s) {
+    int bits = max_bits;
+    if (bits <= 0) {
+        std::vector<u64> ha((size_t)(na * wa)), hb((size_t)(nb * wb));
+        hipError_t he = hipMemcpy(ha.data(), a, ha.size() * 8, hipMemcpyDeviceToHost);
+        if (he == hipSuccess) he = hipMemcpy(hb.data(), b, hb.size() * 8, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "reading inputs", he);
+        bits = std::max(tc_bit_width(ha.data(), na, wa), tc_bit_width(hb.data(), nb, wb));
+    }
+    if (64 * wc < tc_out_bits(bits, bits, na, nb))
+        return fail(CUMODP_ASSUMPTION_ERROR, "output words too narrow for the product");
+    int err = 0;
+    hipError_t he = launch_two_convolution((const u64 *)a, na, wa, (const u64 *)b, nb, wb,
+                                           (u64 *)c, wc, bits, s, &err);
+    if (err) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "no two-convolution base fits these sizes");
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "two-convolution multiply", he);
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err cumodp_two_convolution_poly_mul_dev(const uint64_t *a, int64_t na, int wa,
+                                               const uint64_t *b, int64_t nb, int wb, uint64_t *c,
+                                               int wc, int max_bits, void *stream) {
+    if (na < 1 || nb < 1 || wa < 1 || wb < 1 || wc < 1)
+        return fail(CUMODP_ASSUMPTION_ERROR, "empty polynomial or word count");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return tc_dev_impl(a, na, wa, b, nb, wb, c, wc, max_bits, as_stream(stream));
+}
+
+cumodp_err cumodp_two_convolution_poly_mul(const uint64_t *a, int64_t na, int wa,
+                                           const uint64_t *b, int64_t nb, int wb, uint64_t *c,
+                                           int wc) {
+    if (na < 1 || nb < 1 || wa < 1 || wb < 1 || wc < 1)
+        return fail(CUMODP_ASSUMPTION_ERROR, "empty polynomial or word count");
+    const int ba = tc_bit_width((const u64 *)a, na, wa), bb = tc_bit_width((const u64 *)b, nb, wb);
+    if (64 * wc < tc_out_bits(ba, bb, na, nb))
+        return fail(CUMODP_ASSUMPTION_ERROR, "output words too narrow for the product");
+    const int64_t nc = na + nb - 1;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    u64 *d = (u64 *)workspace(12, sizeof(u64) * (size_t)(na * wa + nb * wb + nc * wc));
+    if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    u64 *da = d, *db = da + na * wa, *dc = db + nb * wb;
+    hipError_t he = hipMemcpy(da, a, sizeof(u64) * (size_t)(na * wa), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(db, b, sizeof(u64) * (size_t)(nb * wb), hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    cumodp_err ce = tc_dev_impl(da, na, wa, db, nb, wb, dc, wc, std::max(ba, bb), nullptr);
+    if (ce != CUMODP_SUCCESS) return ce;
+    he = hipMemcpy(c, dc, sizeof(u64) * (size_t)(nc * wc), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    return CUMODP_SUCCESS;
+}
+
+int cumodp_find_largest_bit_width(const uint64_t *a, int64_t na, int wa, const uint64_t *b,
+                                  int64_t nb, int wb) {
+    return std::max(tc_bit_width((const u64 *)a, na, wa), tc_bit_width((const u64 *)b, nb, wb));
+}
+
+cumodp_err cumodp_convert_to_modular_bivariate(const uint64_t *a, int64_t na, int wa, int K,
+                                               int M, sfixn prime, sfixn *out) {
+    if (K < 1 || M < 1 || M > 62 || prime < 2 || wa < 1)
+        return fail(CUMODP_ASSUMPTION_ERROR, "bivariate base");
+    const u32 P = (u32)prime;
+    std::vector<u64> mag((size_t)wa + 1);
+    for (int64_t i = 0; i < na; ++i) {
+        const u64 *x = a + i * wa;
+        const bool neg = (int64_t)x[wa - 1] < 0;
+        u64 carry = 1;
+        for (int q = 0; q < wa; ++q) {
+            u64 v = x[q];
+            if (neg) { v = ~v + carry; carry = (carry && v == 0); }
+            mag[(size_t)q] = v;
+        }
+        mag[(size_t)wa] = 0;
+        for (int j = 0; j < K; ++j) {
+            const int64_t pos = (int64_t)j * M;
+            const int64_t wd = pos >> 6;
+            const int off = (int)(pos & 63);
+            u64 v = 0;
+            if (wd < wa) {
+                v = mag[(size_t)wd] >> off;
+                if (off && wd + 1 <= wa) v |= mag[(size_t)wd + 1] << (64 - off);
+            }
+            v &= (M >= 64) ? ~0ull : ((1ull << M) - 1);
+            const u32 r = (u32)(v % P);
+            out[i * K + j] = (sfixn)(neg ? (r ? P - r : 0u) : r);
+        }
+    }
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err cumodp_scale_x_argument_dev(sfixn *A, int64_t len, int K, sfixn theta, sfixn prime,
+                                       void *stream) {
+    if (K < 1 || prime < 2) return fail(CUMODP_ASSUMPTION_ERROR, "scale_x arguments");
+    std::lock_guard<std::mutex> lk(api_mutex());
+    hipError_t he = launch_scale_x((u32 *)A, len, K, (u32)theta, (u32)prime, as_stream(stream));
+    if (he == hipSuccess) he = hipStreamSynchronize(as_stream(stream));
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "scale_x", he);
+    return CUMODP_SUCCESS;
+}
+
+int cumodp_two_convolution_plan(int bits, int64_t na, int64_t nb, int *nprimes, int *K, int *M) {
+    return tc_plan_info(bits, na, nb, nprimes, K, M);
 }
 
 cumodp_err cumodp_tune_set_split(int k, int k2) {

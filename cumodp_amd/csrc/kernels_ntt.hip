@@ -535,6 +535,31 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     return col_inv(k1, sh, io.cnt, k2, fam, s);
 }
 
+template <int LOGL>
+static hipError_t rows_mul_t(u32 *F, u32 *G, int rows, const Family *fam, hipStream_t s) {
+    constexpr int C = rows_per_wg<LOGL>();
+    using TL = Tile<LOGL, C, false>;
+    const int nb = (rows + C - 1) / C;
+    ProfScope ps(kProfMulLds, s);
+    hipLaunchKernelGGL((k_mul_lds<LOGL, C, false>), dim3(nb), dim3(TL::T), 0, s, F, G, F, nullptr,
+                       rows, MulDesc{}, fam->lv, fam->lvi, fam->p, fam->pinv,
+                       mul_scale(LOGL, fam->p));
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_mul(int e, u32 *F, u32 *G, int rows, const Family *fam, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    switch (e) {
+#define CM_CASE(K) case K: return rows_mul_t<K>(F, G, rows, fam, s);
+        CM_CASE(4) CM_CASE(5) CM_CASE(6) CM_CASE(7) CM_CASE(8) CM_CASE(9) CM_CASE(10)
+        CM_CASE(11) CM_CASE(12) CM_CASE(13)
+#undef CM_CASE
+    }
+    const int64_t n = (int64_t)1 << e;
+    FourStepIO io{F, G, n, n, n, F, G, F, n, n, rows};
+    return launch_mul_four_step(e, io, fam, s);
+}
+
 hipError_t launch_ntt_four_step(int k, u32 *X, u32 *Y, int cnt, bool inverse, const Family *fam,
                                 hipStream_t s) {
     if (k <= kMaxLdsLog || k > kMaxLog || k > fam->K) return hipErrorInvalidValue;

@@ -54,6 +54,40 @@ typedef struct {
 CUMODP_API cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
                                   const sfixn *G, sfixn *H, sfixn p, void *stream);
 
+/* ---- exact integer polynomial multiplication (two convolutions + CRT) ----
+ * Replaces two_convolution_poly_mul (src/two_convolution_poly_mul.cu:532-569),
+ * whose std::vector<mpz_class> interface needs GMP.  Coefficients are w-word
+ * little-endian two's-complement integers (w = 1 for int64).  c receives
+ * na+nb-1 coefficients of wc words; wc*64 must cover the product's bits
+ * (bits(a) + bits(b) + ceil(log2(min(na, nb))) + 1), else
+ * CUMODP_ASSUMPTION_ERROR.  The result is the exact product. */
+CUMODP_API cumodp_err cumodp_two_convolution_poly_mul(const uint64_t *a, int64_t na, int wa,
+                                                      const uint64_t *b, int64_t nb, int wb,
+                                                      uint64_t *c, int wc);
+/* Same on device buffers, stream ordered.  max_bits: the largest coefficient
+ * bit width of a and b (mpz_sizeinbase semantics); 0 makes the library read
+ * the inputs back to compute it, which synchronises. */
+CUMODP_API cumodp_err cumodp_two_convolution_poly_mul_dev(const uint64_t *a, int64_t na, int wa,
+                                                          const uint64_t *b, int64_t nb, int wb,
+                                                          uint64_t *c, int wc, int max_bits,
+                                                          void *stream);
+/* find_largest_bit_width_of_coefficients (src/two_convolution_poly_mul.cu:58-70):
+ * the largest mpz_sizeinbase(|x|, 2) over a and b (1 for zero) */
+CUMODP_API int cumodp_find_largest_bit_width(const uint64_t *a, int64_t na, int wa,
+                                             const uint64_t *b, int64_t nb, int wb);
+/* convert_to_modular_bivariate (src/two_convolution_poly_mul.cu:84-203): limb j
+ * (bits [jM, jM+M) of |a_i|) of coefficient i at out[i*K + j], reduced mod
+ * prime and negated mod prime when a_i < 0 (canonical residues) */
+CUMODP_API cumodp_err cumodp_convert_to_modular_bivariate(const uint64_t *a, int64_t na, int wa,
+                                                          int K, int M, sfixn prime, sfixn *out);
+/* scale_x_argument_dev (src/two_convolution_poly_mul.cu:207-245) on a plain
+ * device array: A[i] = A[i] * theta^(i mod K) mod prime */
+CUMODP_API cumodp_err cumodp_scale_x_argument_dev(sfixn *A, int64_t len, int K, sfixn theta,
+                                                  sfixn prime, void *stream);
+/* the (primes, K, M) the multiplier uses for these sizes; 0 if none fits */
+CUMODP_API int cumodp_two_convolution_plan(int bits, int64_t na, int64_t nb, int *nprimes,
+                                           int *K, int *M);
+
 /* ---- tuning: four-step split ----------------------------------------------
  * Transforms of 2^k points above 2^13 run as 2^(k-k2) columns x 2^k2 rows;
  * this chooses k2 (12, 13 or 14) for one k.  The tuner (cumodp_amd/autotune.py)

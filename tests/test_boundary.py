@@ -90,3 +90,57 @@ def test_oracle_is_not_linked_into_the_product(cm):
     assert "oracle" not in r.stdout
     raw, _ = exported(cm.LIB_PATH)
     assert not any(s.startswith("orc_") for s in raw)
+
+
+# ---- host-side helpers of the integer multiplier (no device call) -------------
+def _kats():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "tc_kats.json")) as f:
+        return json.load(f)
+
+
+def _words(vals):
+    import oracle
+    m = max(abs(int(v)) for v in vals)
+    W = max(1, (m.bit_length() + 1 + 63) // 64)
+    return oracle.ints_to_words([int(v) for v in vals], W)
+
+
+def test_find_largest_bit_width_kats(cm):
+    """unit_tests/test_two_convolution_poly_mul.cpp:9-54 through the library."""
+    for c in _kats()["bit_width"]:
+        a = _words([int(x) for x in c["a"]])
+        b = _words([int(x) for x in c["b"]])
+        got = cm.lib().cumodp_find_largest_bit_width(cm._ptr(a), a.shape[0], a.shape[1],
+                                                     cm._ptr(b), b.shape[0], b.shape[1])
+        assert got == c["bits"], c
+
+
+def test_convert_to_modular_bivariate_kats(cm):
+    """unit_tests/test_two_convolution_poly_mul.cpp:56-78 through the library."""
+    for c in _kats()["limb_split"]:
+        a = _words(c["a"])
+        out = np.zeros(a.shape[0] * c["K"], np.int32)
+        rc = cm.lib().cumodp_convert_to_modular_bivariate(cm._ptr(a), a.shape[0], a.shape[1],
+                                                          c["K"], c["M"], c["prime"],
+                                                          cm._ptr(out))
+        assert rc == 0 and [int(x) for x in out] == c["expected"], c
+
+
+def test_two_convolution_plan_choices(cm):
+    # 34-bit coefficients of degree 16383: two primes, K = 2 limbs of 17 bits
+    assert cm.two_convolution_plan(34, 16384, 16384) == (2, 2, 17)
+    # 64-bit coefficients, 65536 terms: two primes, four 16-bit limbs
+    assert cm.two_convolution_plan(64, 65536, 65536) == (2, 4, 16)
+    # 1024-bit coefficients, 1024 terms
+    npr, K, M = cm.two_convolution_plan(1024, 1024, 1024)
+    assert K * M >= 1024 and M <= 30
+
+
+def test_two_convolution_rejects_narrow_output(cm):
+    a = np.array([2 ** 40, -3], np.int64)
+    out = np.zeros(3, np.uint64)
+    A = cm._words(a)
+    rc = cm.lib().cumodp_two_convolution_poly_mul(cm._ptr(A), 2, 1, cm._ptr(A), 2, 1,
+                                                  cm._ptr(out), 1)
+    assert rc == cm.CUMODP_ASSUMPTION_ERROR
