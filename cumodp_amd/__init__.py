@@ -143,6 +143,14 @@ def lib():
         L.cumodp_two_convolution_plan.argtypes = [s, i64, i64, C.POINTER(s), C.POINTER(s),
                                                   C.POINTER(s)]
         L.cumodp_two_convolution_plan.restype = s
+        L.cumodp_subproduct_tree_words.argtypes = [s]
+        L.cumodp_subproduct_tree_words.restype = i64
+        L.cumodp_subproduct_tree_offset.argtypes = [s, s]
+        L.cumodp_subproduct_tree_offset.restype = i64
+        L.cumodp_subproduct_tree_dev.argtypes = [vp, s, s, vp, vp]
+        L.cumodp_subproduct_tree_dev.restype = err
+        L.cumodp_subproduct_tree.argtypes = [vp, s, s, vp]
+        L.cumodp_subproduct_tree.restype = err
         _LIB = L
     return _LIB
 
@@ -321,3 +329,20 @@ def two_convolution_plan(bits, na, nb):
     if not lib().cumodp_two_convolution_plan(bits, na, nb, C.byref(npr), C.byref(K), C.byref(M)):
         return None
     return npr.value, K.value, M.value
+
+
+# ---- subproduct tree ----------------------------------------------------------
+def subproduct_tree(points, p):
+    """All levels of the subproduct tree over len(points) = 2^k points mod p:
+    a list whose entry i is a (2^(k-i), 2^i + 1) array of monic products."""
+    pts = _i32(points)
+    k = len(pts).bit_length() - 1
+    assert len(pts) == 1 << k
+    L = lib()
+    tree = np.zeros(L.cumodp_subproduct_tree_words(k), np.int32)
+    _check("cumodp_subproduct_tree", L.cumodp_subproduct_tree(_ptr(pts), k, p, _ptr(tree)))
+    out = []
+    for i in range(k + 1):
+        off = L.cumodp_subproduct_tree_offset(k, i)
+        out.append(tree[off:off + (1 << k)  + (1 << (k - i))].reshape(1 << (k - i), (1 << i) + 1))
+    return out

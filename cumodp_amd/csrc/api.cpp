@@ -471,6 +471,50 @@ int cumodp_two_convolution_plan(int bits, int64_t na, int64_t nb, int *nprimes, 
     return tc_plan_info(bits, na, nb, nprimes, K, M);
 }
 
+// ---- subproduct tree -------------------------------------------------------
+int64_t cumodp_subproduct_tree_words(int k) {
+    return (k < 0 || k > 30) ? -1 : tree_level_offset(k, k + 1);
+}
+
+int64_t cumodp_subproduct_tree_offset(int k, int i) {
+    return (k < 0 || k > 30 || i < 0 || i > k) ? -1 : tree_level_offset(k, i);
+}
+
+static cumodp_err tree_check(int k, sfixn p) {
+    if (k < 1) return fail(CUMODP_ASSUMPTION_ERROR, "tree needs 2^k points, k >= 1");
+    return check_prime_size((u32)p, k < 4 ? 4 : k);
+}
+
+cumodp_err cumodp_subproduct_tree_dev(const sfixn *points, int k, sfixn p, sfixn *tree,
+                                      void *stream) {
+    cumodp_err ce = tree_check(k, p);
+    if (ce != CUMODP_SUCCESS) return ce;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const Family *fam = family_default((u32)p);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    hipError_t he = launch_subproduct_tree((const u32 *)points, k, (u32)p, (u32 *)tree, fam,
+                                           as_stream(stream));
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "subproduct tree", he);
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p, sfixn *tree) {
+    cumodp_err ce = tree_check(k, p);
+    if (ce != CUMODP_SUCCESS) return ce;
+    const int64_t n = (int64_t)1 << k, words = tree_level_offset(k, k + 1);
+    std::lock_guard<std::mutex> lk(api_mutex());
+    u32 *d = (u32 *)workspace(13, sizeof(u32) * (size_t)(n + words));
+    if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    const Family *fam = family_default((u32)p);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    hipError_t he = hipMemcpy(d, points, sizeof(u32) * (size_t)n, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = launch_subproduct_tree(d, k, (u32)p, d + n, fam, 0);
+    if (he == hipSuccess)
+        he = hipMemcpy(tree, d + n, sizeof(u32) * (size_t)words, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "subproduct tree", he);
+    return CUMODP_SUCCESS;
+}
+
 cumodp_err cumodp_tune_set_split(int k, int k2) {
     return set_four_step_split(k, k2) ? CUMODP_SUCCESS
                                       : fail(CUMODP_ASSUMPTION_ERROR, "unsupported four-step split");

@@ -88,6 +88,19 @@ CUMODP_API cumodp_err cumodp_scale_x_argument_dev(sfixn *A, int64_t len, int K, 
 CUMODP_API int cumodp_two_convolution_plan(int bits, int64_t na, int64_t nb, int *nprimes,
                                            int *K, int *M);
 
+/* ---- subproduct tree ------------------------------------------------------
+ * onlySubproductTree / subProductTree (src/fastPolyEvaluation.cu:512-656,
+ * :1411-1434) over 2^k points mod p.  Level i (0 <= i <= k) holds the 2^(k-i)
+ * monic products of 2^i consecutive (x - x_j), each stored with its leading
+ * coefficient as 2^i + 1 canonical words, at word offset
+ * cumodp_subproduct_tree_offset(k, i); level k is the root (the reference does
+ * not form it).  The whole tree takes cumodp_subproduct_tree_words(k) words. */
+CUMODP_API int64_t cumodp_subproduct_tree_words(int k);
+CUMODP_API int64_t cumodp_subproduct_tree_offset(int k, int i);
+CUMODP_API cumodp_err cumodp_subproduct_tree_dev(const sfixn *points, int k, sfixn p, sfixn *tree,
+                                                 void *stream);
+CUMODP_API cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p, sfixn *tree);
+
 /* ---- tuning: four-step split ----------------------------------------------
  * Transforms of 2^k points above 2^13 run as 2^(k-k2) columns x 2^k2 rows;
  * this chooses k2 (12, 13 or 14) for one k.  The tuner (cumodp_amd/autotune.py)
