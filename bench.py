@@ -72,12 +72,113 @@ def cpu_baseline(F, G, p, reps=2):
     return H, t
 
 
+def timed(fn, reps, sync):
+    """Median wall time of fn() over reps runs, each bracketed by sync()."""
+    ts = []
+    for _ in range(reps):
+        sync()
+        t0 = time.perf_counter()
+        fn()
+        sync()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def bench_list_multiply(cm, reps=5):
+    """4096 pairs of uneven degree (log-uniform in [32, 16384]) mod 469762049:
+    one batched cumodp_list_fftmul_dev call against one cumodp_fftmul_uni_dev
+    call per pair, on the same resident inputs."""
+    w = cm.workload4()
+    p = w["p"]
+    items, nf, ng, nh = cm.make_items(w["df"], w["dg"])
+    dF, dG = cm.DeviceArray.from_host(w["F"]), cm.DeviceArray.from_host(w["G"])
+    dH1, dH2 = cm.DeviceArray(nh), cm.DeviceArray(nh)
+    run_b = lambda: cm.list_fftmul_dev(items, dF, dG, dH1, p)
+    run_c = lambda: cm.list_fftmul_dev(items, dF, dG, dH2, p, per_call=True)
+    run_b(), run_c()
+    tb = timed(run_b, reps, cm.synchronize)
+    tc = timed(run_c, reps, cm.synchronize)
+    H1, H2 = dH1.download(), dH2.download()
+    # spot check three pairs against the oracle's schoolbook product
+    import oracle
+    ok = bool(np.array_equal(H1, H2))
+    for i in (0, len(items) // 2, len(items) - 1):
+        it = items[i]
+        f = w["F"][it.f_off:it.f_off + it.df + 1]
+        g = w["G"][it.g_off:it.g_off + it.dg + 1]
+        ok = ok and bool(np.array_equal(H1[it.h_off:it.h_off + it.df + it.dg + 1],
+                                        oracle.plain_polymul(f, g, p)))
+    for d in (dF, dG, dH1, dH2):
+        d.free()
+    return {"workload": "4096 pairs, degrees log-uniform [32, 16384], mod 469762049",
+            "output_coeffs": nh,
+            "batched": {"ms": tb * 1e3, "coeffs_per_s": nh / tb},
+            "per_call": {"ms": tc * 1e3, "coeffs_per_s": nh / tc},
+            "batched_over_per_call": tc / tb, "check_batched_eq_per_call_eq_oracle": ok}
+
+
+def bench_tree(cm, reps=5):
+    """Subproduct tree over 2^16 points mod 469762049, all 17 levels."""
+    w = cm.workload4_tree()
+    p, pts = w["p"], w["points"]
+    k = 16
+    L = cm.lib()
+    nw = L.cumodp_subproduct_tree_words(k)
+    dP, dT = cm.DeviceArray.from_host(pts), cm.DeviceArray(nw)
+    run = lambda: cm._check("cumodp_subproduct_tree_dev", L.cumodp_subproduct_tree_dev(
+        dP.addr(), k, p, dT.addr(), None))
+    run()
+    t = timed(run, reps, cm.synchronize)
+    T = dT.download(count=(1 << k) + 1, offset=L.cumodp_subproduct_tree_offset(k, k))
+    r = 123456789
+    ok = eval_mod(T, r, p) == _prod_mod([(r - int(x)) % p for x in pts], p)
+    dP.free(), dT.free()
+    return {"workload": "subproduct tree, 2^16 points uniform mod 469762049, levels 0..16",
+            "ms": t * 1e3, "tree_words": nw, "check_root_evaluation": bool(ok)}
+
+
+def _prod_mod(vals, p):
+    acc = 1
+    for v in vals:
+        acc = acc * v % p
+    return acc
+
+
+def bench_two_convolution(cm, reps=5):
+    """Exact product of two degree-16383 int64 polynomials with coefficients
+    uniform in (-10^10, 10^10) (34 bits); output as 2-word integers."""
+    w = cm.workload3()
+    a, b = w["a"], w["b"]
+    A, B = cm._words(a, 1), cm._words(b, 1)
+    na, nb = len(a), len(b)
+    nc, wc = na + nb - 1, 2
+    L = cm.lib()
+    dA, dB = cm.DeviceArray.from_host(A.ravel()), cm.DeviceArray.from_host(B.ravel())
+    dC = cm.DeviceArray(nc * wc, np.uint64)
+    bits = L.cumodp_find_largest_bit_width(cm._ptr(A), na, 1, cm._ptr(B), nb, 1)
+    run = lambda: cm._check("cumodp_two_convolution_poly_mul_dev",
+                            L.cumodp_two_convolution_poly_mul_dev(
+                                dA.addr(), na, 1, dB.addr(), nb, 1, dC.addr(), wc, bits, None))
+    run()
+    t = timed(run, reps, cm.synchronize)
+    Cw = dC.download().reshape(nc, wc)
+    import oracle
+    ok = bool(np.array_equal(Cw, oracle.zmul_i64(a, b)[:, :wc]))
+    dA.free(), dB.free(), dC.free()
+    return {"workload": "two degree-16383 int64 polynomials, coefficients uniform in "
+                        "(-10^10, 10^10), exact product", "plan": cm.two_convolution_plan(
+                            bits, na, nb),
+            "ms": t * 1e3, "coeffs_per_s": nc / t, "check_exact_vs_schoolbook": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--primary-only", action="store_true",
+                    help="skip the list-multiply, tree and integer-multiply lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is single-GPU); got --gpus {args.gpus}",
@@ -171,6 +272,15 @@ def main():
                                          "median of 2 runs of the oracle's serial NTT multiply",
                                "seconds_per_multiply": tc}
     L.cumodp_stream_destroy(stream)
+    dF.free(), dG.free(), dH.free()
+    if not args.primary_only:
+        sec = {}
+        for key, fn in (("list_multiply", bench_list_multiply), ("subproduct_tree", bench_tree),
+                        ("two_convolution", bench_two_convolution)):
+            log(f"bench: {key}")
+            sec[key] = fn(cm)
+            log(f"bench: {key}: {sec[key]}")
+        out["secondary"] = sec
     print(json.dumps(out), flush=True)
 
 

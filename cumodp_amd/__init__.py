@@ -116,6 +116,9 @@ def lib():
         L.cumodp_fftmul_uni_dev.restype = err
         L.cumodp_list_fftmul_dev.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp, s, vp]
         L.cumodp_list_fftmul_dev.restype = err
+        L.cumodp_list_fftmul_dev_percall.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp,
+                                                     s, vp]
+        L.cumodp_list_fftmul_dev_percall.restype = err
         L.cumodp_fft_bivariate.argtypes = [sp, s, s, s, s, s]
         L.cumodp_fft_bivariate.restype = err
         L.cumodp_invfft_bivariate.argtypes = [sp, s, s, s, s, s]
@@ -276,10 +279,28 @@ def fftmul_uni_dev(dh, Hd, df, Fd, dg, Gd, p, stream=None):
         dh, a(Hd), df, a(Fd), dg, a(Gd), p, C.c_void_p(stream)))
 
 
-def list_fftmul_dev(items, Fd, Gd, Hd, p, stream=None):
-    """cumodp_list_fftmul_dev: items is a ctypes array of MulItem."""
-    _check("cumodp_list_fftmul_dev", lib().cumodp_list_fftmul_dev(
-        items, len(items), Fd.addr(), Gd.addr(), Hd.addr(), p, C.c_void_p(stream)))
+def list_fftmul_dev(items, Fd, Gd, Hd, p, stream=None, per_call=False):
+    """cumodp_list_fftmul_dev (or the per-call form): items is a ctypes array
+    of MulItem."""
+    fn = "cumodp_list_fftmul_dev_percall" if per_call else "cumodp_list_fftmul_dev"
+    _check(fn, getattr(lib(), fn)(items, len(items), Fd.addr(), Gd.addr(), Hd.addr(), p,
+                                  C.c_void_p(stream)))
+
+
+def make_items(df, dg):
+    """Packed layout for pairs of degrees df[i], dg[i]: ctypes MulItem array
+    and the total F, G, H word counts."""
+    n = len(df)
+    items = (MulItem * n)()
+    fo = go = ho = 0
+    for i in range(n):
+        a, b = int(df[i]), int(dg[i])
+        items[i].f_off, items[i].g_off, items[i].h_off = fo, go, ho
+        items[i].df, items[i].dg = a, b
+        fo += a + 1
+        go += b + 1
+        ho += a + b + 1
+    return items, fo, go, ho
 
 
 # ---- optional per-kernel timing ----------------------------------------------

@@ -547,6 +547,20 @@ cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, c
     return CUMODP_SUCCESS;
 }
 
+// One cumodp_fftmul_uni_dev-equivalent call per item (the plain per-call form
+// a caller would write as a loop), for comparison with the batched engine.
+cumodp_err cumodp_list_fftmul_dev_percall(const cumodp_mul_item *items, int64_t count,
+                                          const sfixn *F, const sfixn *G, sfixn *H, sfixn p,
+                                          void *stream) {
+    for (int64_t i = 0; i < count; ++i) {
+        const cumodp_mul_item &it = items[i];
+        cumodp_err ce = cumodp_fftmul_uni_dev(it.df + it.dg, H + it.h_off, it.df, F + it.f_off,
+                                              it.dg, G + it.g_off, p, stream);
+        if (ce != CUMODP_SUCCESS) return ce;
+    }
+    return CUMODP_SUCCESS;
+}
+
 }  // extern "C"
 
 // ===========================================================================

@@ -53,6 +53,11 @@ typedef struct {
 
 CUMODP_API cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
                                   const sfixn *G, sfixn *H, sfixn p, void *stream);
+/* The same products as one cumodp_fftmul_uni_dev call per item (the plain
+ * per-call form), kept for comparisons with the batched engine. */
+CUMODP_API cumodp_err cumodp_list_fftmul_dev_percall(const cumodp_mul_item *items, int64_t count,
+                                                     const sfixn *F, const sfixn *G, sfixn *H,
+                                                     sfixn p, void *stream);
 
 /* ---- exact integer polynomial multiplication (two convolutions + CRT) ----
  * Replaces two_convolution_poly_mul (src/two_convolution_poly_mul.cu:532-569),
