@@ -252,7 +252,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic (SplitMix64 seed 2, uniform residues)",
         "config": {"workload": "one NTT multiply of two degree-(2^23-1) polynomials mod "
-                               "2013265921, transform size 2^24 (four-step 2^12 x 2^12)",
+                               "2013265921, transform size 2^24 (four-step 2^11 columns x 2^13 rows)",
                    "n": n, "p": p},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -116,11 +116,7 @@ cumodp_err ntt_dev_impl(u32 *X, int64_t m, int k, u32 w, u32 p, bool inverse, hi
     if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
     hipError_t he;
     if (k <= kMaxLdsLog) {
-        uint2 sc = make_uint2(1u, 0u);
-        if (inverse) {
-            u32 ni = cm_host_inv(cm_host_pow(2, k, p), p);
-            sc = make_uint2(ni, cm_host_shoup_q(ni, p));
-        }
+        const u32 sc = inverse ? cm_host_inv(cm_host_pow(2, k, p), p) : 1u;
         he = launch_ntt_lds(k, X, m, inverse, sc, fam, s);
     } else {
         const int64_t n = (int64_t)1 << k;
@@ -162,11 +158,7 @@ cumodp_err biv_dev_impl(u32 *X, int em, u32 wm, int en, u32 wn, u32 p, bool inve
     if (ce != CUMODP_SUCCESS) return ce;
     const Family *fm = family_for_root(p, wm % p, em);
     if (!fm) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
-    uint2 sc = make_uint2(1u, 0u);
-    if (inverse) {
-        u32 ni = cm_host_inv(cm_host_pow(2, em + en, p), p);
-        sc = make_uint2(ni, cm_host_shoup_q(ni, p));
-    }
+    const u32 sc = inverse ? cm_host_inv(cm_host_pow(2, em + en, p), p) : 1u;
     hipError_t he = launch_col_ntt(em, X, (int64_t)1 << en, sc, fm, s);
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "bivariate transform", he);

@@ -30,12 +30,16 @@ struct TileDesc {
 constexpr int kMixT = 256;          // threads of the mixed-size workgroup
 constexpr int kMixWords = 4096;     // words per tile
 
+struct Scales {
+    u32 v[16];
+};
+
 template <int LOGL>
 using MixTile = Tile<LOGL, (kMixWords >> LOGL), false, kMixT>;
 
 template <int LOGL>
 CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const MulDesc *d,
-                     int cnt, const uint2 *lv, const uint2 *lvi, u32 p, u32 pinv, uint2 sc) {
+                     int cnt, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
     using TL = MixTile<LOGL>;
     constexpr int L = TL::L;
     // element idx of the tile: vector c = idx >> LOGL, position i; all loads
@@ -60,7 +64,7 @@ CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const M
         B[TL::addr(c, i)] = ga[r];
     }
     __syncthreads();
-    cm_tile_multiply<TL>(A, B, lv, lvi, p, pinv, sc);
+    cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
         const int c = idx >> LOGL, i = idx & (L - 1);
         if (c < cnt && i < d[c].lh) H[d[c].h + i] = A[TL::addr(c, i)];
@@ -70,24 +74,23 @@ CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const M
 __global__ __launch_bounds__(kMixT) void k_mul_mixed(
     const u32 *__restrict__ F, const u32 *__restrict__ G, u32 *__restrict__ H,
     const MulDesc *__restrict__ descs, const TileDesc *__restrict__ tiles,
-    const uint2 *__restrict__ lv, const uint2 *__restrict__ lvi, u32 p, u32 pinv,
-    const uint2 *__restrict__ scales) {
+    const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md, Scales scales) {
     constexpr int W = kMixWords + (kMixWords >> 4);
     __shared__ u32 lds[2 * W];
     u32 *A = lds, *B = lds + W;
     const TileDesc t = tiles[blockIdx.x];
     const MulDesc *d = descs + t.first;
-    const uint2 sc = scales[t.logn];
+    const u32 sc = scales.v[t.logn];
     switch (t.logn) {
-    case 4: mul_tile<4>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 5: mul_tile<5>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 6: mul_tile<6>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 7: mul_tile<7>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 8: mul_tile<8>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 9: mul_tile<9>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 10: mul_tile<10>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 11: mul_tile<11>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
-    case 12: mul_tile<12>(A, B, F, G, H, d, t.count, lv, lvi, p, pinv, sc); break;
+    case 4: mul_tile<4>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 5: mul_tile<5>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 6: mul_tile<6>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 7: mul_tile<7>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 8: mul_tile<8>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 9: mul_tile<9>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 10: mul_tile<10>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 11: mul_tile<11>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+    case 12: mul_tile<12>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
     default: break;
     }
 }
@@ -152,24 +155,11 @@ hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s)
     return hipEventRecord(st.done, s);
 }
 
-const uint2 *scale_table(const Family *fam) {
-    // 2^32 / 2^e mod p for e = 0 .. 13, one device table per family
-    static std::mutex m;
-    static std::vector<std::pair<const Family *, uint2 *>> tabs;
-    std::lock_guard<std::mutex> lk(m);
-    for (auto &t : tabs)
-        if (t.first == fam) return t.second;
-    uint2 h[16];
-    for (int e = 0; e < 16; ++e) {
-        u32 ninv = cm_host_inv(cm_host_pow(2, e, fam->p), fam->p);
-        u32 c = cm_host_mul(cm_host_r1(fam->p), ninv, fam->p);
-        h[e] = make_uint2(c, cm_host_shoup_q(c, fam->p));
-    }
-    uint2 *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(h)) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-    tabs.push_back({fam, d});
-    return d;
+// pointwise constants of the 2^e-point products, e = 0 .. 15
+Scales scale_table(const Family *fam) {
+    Scales t;
+    for (int e = 0; e < 16; ++e) t.v[e] = mul_scale(e, fam->p);
+    return t;
 }
 
 }  // namespace
@@ -211,11 +201,10 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
     const TileDesc *d_tiles = (const TileDesc *)(dblob + desc_bytes);
 
     if (!tiles.empty()) {
-        const uint2 *sc = scale_table(fam);
-        if (!sc) return hipErrorOutOfMemory;
         ProfScope ps(kProfMulMixed, s);
         hipLaunchKernelGGL(k_mul_mixed, dim3((unsigned)tiles.size()), dim3(kMixT), 0, s, F, G, H,
-                           d_descs, d_tiles, fam->lv, fam->lvi, fam->p, fam->pinv, sc);
+                           d_descs, d_tiles, fam->lv, fam->lvi, Mod{fam->p, fam->np},
+                           scale_table(fam));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -33,8 +33,8 @@ CM_DEV int xcd_remap(int b, int nb) {
 template <int LOGL, int C, bool DESC>
 __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
     const u32 *F, const u32 *__restrict__ G, u32 *H, const MulDesc *__restrict__ descs,
-    int ndesc, MulDesc one, const uint2 *__restrict__ lv, const uint2 *__restrict__ lvi, u32 p,
-    u32 pinv, uint2 sc) {
+    int ndesc, MulDesc one, const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md,
+    u32 sc) {
     using TL = Tile<LOGL, C, false>;
     __shared__ u32 lds[2 * TL::LDS_WORDS];
     u32 *A = lds, *B = lds + TL::LDS_WORDS;
@@ -75,7 +75,7 @@ __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
         B[TL::addr(c, i)] = ga[r];
     }
     __syncthreads();
-    cm_tile_multiply<TL>(A, B, lv, lvi, p, pinv, sc);
+    cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
@@ -92,7 +92,7 @@ __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
 // ---------------------------------------------------------------------------
 template <int LOGL, int C>
 __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_ntt_lds(
-    u32 *__restrict__ X, int64_t m, const uint2 *__restrict__ lv, u32 p, uint2 sc) {
+    u32 *__restrict__ X, int64_t m, const u32 *__restrict__ lv, Mod md, u32 sc) {
     using TL = Tile<LOGL, C, false>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int L = TL::L;
@@ -110,14 +110,13 @@ __global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_ntt_lds(
         lds[TL::addr(c, i)] = xa[r];
     }
     __syncthreads();
-    cm_tile_dif<TL>(lds, lv, p);
-    const bool scale = sc.x != 1u;
+    cm_tile_dif<TL>(lds, lv, md);
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         if (r0 + c >= m) break;
         for (int i = threadIdx.x; i < L; i += TL::T) {
             u32 v = lds[TL::addr(c, (int)cm_brev(i, LOGL))];
-            if (scale) v = cm_shoup(v, sc.x, sc.y, p);
+            if (sc) v = cm_mul_tw(v, sc, md.p, md.np);
             X[(r0 + c) * L + i] = v;
         }
     }
@@ -141,8 +140,8 @@ struct ColSet {
 // s0, the rest from s1).
 template <int LOGL1, int C, bool OUTER>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
-    ColSet s0, ColSet s1, int cnt0, int logn2, const uint2 *__restrict__ lv,
-    const uint2 *__restrict__ olo, const uint2 *__restrict__ ohi, u32 p) {
+    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv,
+    const u32 *__restrict__ olo, const u32 *__restrict__ ohi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int y = blockIdx.y;
@@ -163,15 +162,15 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
 #pragma unroll
     for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
-    cm_tile_dif<TL>(lds, lv, p);
+    cm_tile_dif<TL>(lds, lv, md);
     constexpr int LOM = (1 << kOuterLoBits) - 1;
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
         u32 v = lds[cm_pad(idx)];
         if (OUTER) {
             const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
-            const uint2 wa = olo[e & LOM], wb = ohi[e >> kOuterLoBits];
-            v = cm_shoup(cm_shoup_lazy(v, wa.x, wa.y, p), wb.x, wb.y, p);
+            v = cm_mul_tw(cm_mont_lazy(v, olo[e & LOM], md.p, md.np), ohi[e >> kOuterLoBits],
+                          md.p, md.np);
         }
         Y[i * n2 + c0 + c] = v;
     }
@@ -182,7 +181,7 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
 // pass.  grid.y: arrays.
 template <int LOGL1, int C>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_inv(
-    ColSet S, int logn2, const uint2 *__restrict__ lvi, u32 p) {
+    ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int64_t a = blockIdx.y;
@@ -200,7 +199,7 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_inv(
 #pragma unroll
     for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
-    cm_tile_dit<TL>(lds, lvi, p);
+    cm_tile_dit<TL>(lds, lvi, md);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
         const int64_t e = i * n2 + c0 + c;
@@ -218,9 +217,9 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_inv(
 template <int LOGL2>
 __global__ __launch_bounds__((Tile<LOGL2, 1, false>::T))
 __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
-    u32 *F, const u32 *__restrict__ G, int logn1, const uint2 *__restrict__ rb,
-    const uint2 *__restrict__ rc, const uint2 *__restrict__ rbi, const uint2 *__restrict__ rci,
-    const uint2 *__restrict__ lv, const uint2 *__restrict__ lvi, u32 p, u32 pinv, uint2 sc) {
+    u32 *F, const u32 *__restrict__ G, int logn1, const u32 *__restrict__ rb,
+    const u32 *__restrict__ rc, const u32 *__restrict__ rbi, const u32 *__restrict__ rci,
+    const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md, u32 sc) {
     using TL = Tile<LOGL2, 1, false>;
     constexpr int T = TL::T;
     static_assert(TL::L == 16 * T, "row pass assumes 16 elements per thread");
@@ -231,7 +230,7 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     u32 *Fr = F + r * TL::L;
     const u32 *Gr = G + r * TL::L;
     const int t = threadIdx.x;
-    const uint2 b = rb[(int64_t)k1 * T + t];
+    const u32 b = rb[(int64_t)k1 * T + t];
     u32 fa[16], ga[16];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
@@ -241,18 +240,18 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int i = t + T * m;
-        const uint2 c = rc[k1 * 16 + m];
-        A[cm_pad(i)] = cm_shoup(cm_shoup_lazy(fa[m], c.x, c.y, p), b.x, b.y, p);
-        B[cm_pad(i)] = cm_shoup(cm_shoup_lazy(ga[m], c.x, c.y, p), b.x, b.y, p);
+        const u32 c = rc[k1 * 16 + m];
+        A[cm_pad(i)] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md.p, md.np);
+        B[cm_pad(i)] = cm_mul_tw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md.p, md.np);
     }
     __syncthreads();
-    cm_tile_multiply<TL>(A, B, lv, lvi, p, pinv, sc);
-    const uint2 bi = rbi[(int64_t)k1 * T + t];
+    cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
+    const u32 bi = rbi[(int64_t)k1 * T + t];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int i = t + T * m;
-        const uint2 c = rci[k1 * 16 + m];
-        Fr[i] = cm_shoup(cm_shoup_lazy(A[cm_pad(i)], c.x, c.y, p), bi.x, bi.y, p);
+        const u32 c = rci[k1 * 16 + m];
+        Fr[i] = cm_mul_tw(cm_mont_lazy(A[cm_pad(i)], c, md.p, md.np), bi, md.p, md.np);
     }
 }
 
@@ -262,7 +261,7 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
 template <int LOGL2, int CR>
 __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
     const u32 *__restrict__ X0, u32 *__restrict__ Y0, int64_t stride, int logn1,
-    const uint2 *__restrict__ lv, u32 p, uint2 sc) {
+    const u32 *__restrict__ lv, Mod md, u32 sc) {
     using TL = Tile<LOGL2, CR, false>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const u32 *X = X0 + (int64_t)blockIdx.y * stride;
@@ -283,12 +282,11 @@ __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
         lds[TL::addr(c, i)] = xa[r];
     }
     __syncthreads();
-    cm_tile_dif<TL>(lds, lv, p);
-    const bool scale = sc.x != 1u;
+    cm_tile_dif<TL>(lds, lv, md);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int k2 = idx / CR, c = idx % CR;
         u32 v = lds[TL::addr(c, (int)cm_brev(k2, LOGL2))];
-        if (scale) v = cm_shoup(v, sc.x, sc.y, p);
+        if (sc) v = cm_mul_tw(v, sc, md.p, md.np);
         Y[k10 + c + k2 * n1] = v;
     }
 }
@@ -298,7 +296,7 @@ __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
 // src/list_stockham.cu:1193-1201); output scaled by sc when sc.x != 1.
 template <int LOGM, int C>
 __global__ __launch_bounds__((Tile<LOGM, C, true>::T)) void k_col_ntt(
-    u32 *__restrict__ X, int64_t ncols, const uint2 *__restrict__ lv, u32 p, uint2 sc) {
+    u32 *__restrict__ X, int64_t ncols, const u32 *__restrict__ lv, Mod md, u32 sc) {
     using TL = Tile<LOGM, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
     const int64_t c0 = (int64_t)blockIdx.x * C;
@@ -312,13 +310,12 @@ __global__ __launch_bounds__((Tile<LOGM, C, true>::T)) void k_col_ntt(
 #pragma unroll
     for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
-    cm_tile_dif<TL>(lds, lv, p);
-    const bool scale = sc.x != 1u;
+    cm_tile_dif<TL>(lds, lv, md);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
         if (c0 + c >= ncols) continue;
         u32 v = lds[cm_pad((int)cm_brev(i, LOGM) * C + c)];
-        if (scale) v = cm_shoup(v, sc.x, sc.y, p);
+        if (sc) v = cm_mul_tw(v, sc, md.p, md.np);
         X[i * ncols + c0 + c] = v;
     }
 }
@@ -367,12 +364,16 @@ static int ew_blocks(int64_t n) {
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static uint2 shoup_pair(u32 w, u32 p) { return make_uint2(w, cm_host_shoup_q(w, p)); }
+static Mod mod_of(const Family *fam) { return Mod{fam->p, fam->np}; }
 
-// Shoup constant folding Montgomery's 2^-32 and 1/N: 2^32 / N mod p.
-static uint2 mul_scale(int logn, u32 p) {
+// Montgomery form of the output scale `scale` (0 when scale == 1: no scaling)
+static u32 scale_mont(u32 scale, u32 p) { return scale % p == 1u ? 0u : cm_host_mont(scale % p, p); }
+
+// Pointwise constant of an N = 2^logn multiply: the Montgomery form of
+// 2^32 / N mod p (see cm_tile_multiply).
+u32 mul_scale(int logn, u32 p) {
     u32 ninv = cm_host_inv(cm_host_pow(2, logn, p), p);
-    return shoup_pair(cm_host_mul(cm_host_r1(p), ninv, p), p);
+    return cm_host_mont(cm_host_mul(cm_host_r1(p), ninv, p), p);
 }
 
 template <int LOGL>
@@ -386,7 +387,7 @@ static hipError_t mul_lds_t(const u32 *F, const u32 *G, u32 *H, const MulDesc *d
     const int nb = (ndesc + C - 1) / C;
     ProfScope ps(kProfMulLds, s);
     hipLaunchKernelGGL((k_mul_lds<LOGL, C, DESC>), dim3(nb), dim3(TL::T), 0, s, F, G, H, descs,
-                       ndesc, one, fam->lv, fam->lvi, fam->p, fam->pinv, mul_scale(LOGL, fam->p));
+                       ndesc, one, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
     return hipGetLastError();
 }
 
@@ -410,21 +411,22 @@ hipError_t launch_mul_lds_one(int logn, const u32 *F, const u32 *G, u32 *H,
 }
 
 template <int LOGL>
-static hipError_t ntt_lds_t(u32 *X, int64_t m, const uint2 *lv, uint2 sc, const Family *fam,
+static hipError_t ntt_lds_t(u32 *X, int64_t m, const u32 *lv, u32 sc, const Family *fam,
                             hipStream_t s) {
     constexpr int C = rows_per_wg<LOGL>();
     using TL = Tile<LOGL, C, false>;
     const int64_t nb = (m + C - 1) / C;
     ProfScope ps(kProfNttLds, s);
     hipLaunchKernelGGL((k_ntt_lds<LOGL, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, m, lv,
-                       fam->p, sc);
+                       mod_of(fam), sc);
     return hipGetLastError();
 }
 
-hipError_t launch_ntt_lds(int k, u32 *X, int64_t m, bool inverse, uint2 sc, const Family *fam,
+hipError_t launch_ntt_lds(int k, u32 *X, int64_t m, bool inverse, u32 scale, const Family *fam,
                           hipStream_t s) {
     if (m <= 0) return hipSuccess;
-    const uint2 *lv = inverse ? fam->lvi : fam->lv;
+    const u32 *lv = inverse ? fam->lvi : fam->lv;
+    const u32 sc = scale_mont(scale, fam->p);
     switch (k) {
 #define CM_CASE(K) case K: return ntt_lds_t<K>(X, m, lv, sc, fam, s);
         CM_CASE(4) CM_CASE(5) CM_CASE(6) CM_CASE(7) CM_CASE(8) CM_CASE(9) CM_CASE(10)
@@ -442,7 +444,7 @@ static constexpr int col_tile() {
 
 template <int LOGL1>
 static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
-                            const uint2 *lv, const uint2 *olo, const uint2 *ohi, u32 p,
+                            const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
                             hipStream_t s) {
     constexpr int C = col_tile<LOGL1>();
     using TL = Tile<LOGL1, C, true>;
@@ -450,10 +452,10 @@ static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     ProfScope ps(kProfColFwd, s);
     if (olo)
         hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0, s,
-                           s0, s1, cnt0, logn2, lv, olo, ohi, p);
+                           s0, s1, cnt0, logn2, lv, olo, ohi, md);
     else
         hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0,
-                           s, s0, s1, cnt0, logn2, lv, olo, ohi, p);
+                           s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
     return hipGetLastError();
 }
 
@@ -465,19 +467,19 @@ static hipError_t col_inv_t(const ColSet &S, int cnt, int logn2, const Family *f
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColInv, s);
     hipLaunchKernelGGL((k_col_inv<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S, logn2,
-                       fam->lvi, fam->p);
+                       fam->lvi, mod_of(fam));
     return hipGetLastError();
 }
 
 template <int LOGL2>
 static hipError_t row_out_t(const u32 *X, u32 *Y, int cnt, int64_t stride, int logn1,
-                            const uint2 *lv, uint2 sc, const Family *fam, hipStream_t s) {
+                            const u32 *lv, u32 sc, const Family *fam, hipStream_t s) {
     constexpr int CR = LOGL2 >= 12 ? (1 << (14 - LOGL2)) : 4;   // 16384 words per tile
     using TL = Tile<LOGL2, CR, false>;
     const int nb = (int)(((int64_t)1 << logn1) / CR);
     ProfScope ps(kProfRowOut, s);
     hipLaunchKernelGGL((k_row_out<LOGL2, CR>), dim3(nb, cnt), dim3(TL::T), 0, s, X, Y, stride,
-                       logn1, lv, fam->p, sc);
+                       logn1, lv, mod_of(fam), sc);
     return hipGetLastError();
 }
 
@@ -494,9 +496,9 @@ static hipError_t row_out_t(const u32 *X, u32 *Y, int cnt, int64_t stride, int l
     return hipErrorInvalidValue;
 
 static hipError_t col_fwd(int k1, const ColSet &s0, const ColSet &s1, int cnt0, int cnt1,
-                          int logn2, const uint2 *lv, const uint2 *olo, const uint2 *ohi, u32 p,
+                          int logn2, const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
                           hipStream_t s) {
-    CM_K1_CASES(col_fwd_t, s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, p, s)
+    CM_K1_CASES(col_fwd_t, s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, md, s)
 }
 
 static hipError_t col_inv(int k1, const ColSet &S, int cnt, int logn2, const Family *fam,
@@ -513,17 +515,18 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     const int64_t n = (int64_t)1 << k;
     ColSet sf{io.f_in, io.f_ws, io.in_stride, io.f_len, n, n};
     ColSet sg{io.g_in, io.g_ws, io.in_stride, io.g_len, n, n};
-    hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr, fam->p, s);
+    hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr,
+                           mod_of(fam), s);
     if (e != hipSuccess) return e;
     // row pass: one row of F and G per workgroup (2^k1 * cnt rows)
     const int nrows = (int)(((int64_t)io.cnt << k1));
-    const uint2 sc = mul_scale(k, fam->p);
+    const u32 sc = mul_scale(k, fam->p);
     {
         ProfScope ps(kProfRowMul, s);
 #define CM_ROW(K2)                                                                         \
     hipLaunchKernelGGL((k_row_mul<K2>), dim3(nrows), dim3(Tile<K2, 1, false>::T), 0, s,    \
                        io.f_ws, io.g_ws, k1, o->rb_f, o->rc_f, o->rb_i, o->rc_i, fam->lv,  \
-                       fam->lvi, fam->p, fam->pinv, sc)
+                       fam->lvi, mod_of(fam), sc)
         if (k2 == 12) CM_ROW(12);
         else if (k2 == 13) CM_ROW(13);
         else CM_ROW(14);
@@ -542,8 +545,7 @@ static hipError_t rows_mul_t(u32 *F, u32 *G, int rows, const Family *fam, hipStr
     const int nb = (rows + C - 1) / C;
     ProfScope ps(kProfMulLds, s);
     hipLaunchKernelGGL((k_mul_lds<LOGL, C, false>), dim3(nb), dim3(TL::T), 0, s, F, G, F, nullptr,
-                       rows, MulDesc{}, fam->lv, fam->lvi, fam->p, fam->pinv,
-                       mul_scale(LOGL, fam->p));
+                       rows, MulDesc{}, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
     return hipGetLastError();
 }
 
@@ -569,13 +571,13 @@ hipError_t launch_ntt_four_step(int k, u32 *X, u32 *Y, int cnt, bool inverse, co
     if (!o) return hipErrorOutOfMemory;
     const int64_t n = (int64_t)1 << k;
     // the inverse transform is the forward transform with the inverse root
-    const uint2 *lv = inverse ? fam->lvi : fam->lv;
-    const uint2 *olo = inverse ? o->lo_i : o->lo_f, *ohi = inverse ? o->hi_i : o->hi_f;
+    const u32 *lv = inverse ? fam->lvi : fam->lv;
+    const u32 *olo = inverse ? o->lo_i : o->lo_f, *ohi = inverse ? o->hi_i : o->hi_f;
     ColSet sx{X, X, n, n, n, n};
-    hipError_t e = col_fwd(k1, sx, sx, cnt, 0, k2, lv, olo, ohi, fam->p, s);
+    hipError_t e = col_fwd(k1, sx, sx, cnt, 0, k2, lv, olo, ohi, mod_of(fam), s);
     if (e != hipSuccess) return e;
-    uint2 sc = make_uint2(1u, 0u);
-    if (inverse) sc = shoup_pair(cm_host_inv(cm_host_pow(2, k, fam->p), fam->p), fam->p);
+    const u32 sc =
+        inverse ? scale_mont(cm_host_inv(cm_host_pow(2, k, fam->p), fam->p), fam->p) : 0u;
     if (k2 == 12) e = row_out_t<12>(X, Y, cnt, n, k1, lv, sc, fam, s);
     else if (k2 == 13) e = row_out_t<13>(X, Y, cnt, n, k1, lv, sc, fam, s);
     else e = row_out_t<14>(X, Y, cnt, n, k1, lv, sc, fam, s);
@@ -584,19 +586,20 @@ hipError_t launch_ntt_four_step(int k, u32 *X, u32 *Y, int cnt, bool inverse, co
 }
 
 template <int LOGM>
-static hipError_t col_ntt_t(u32 *X, int64_t ncols, uint2 sc, const Family *fam, hipStream_t s) {
+static hipError_t col_ntt_t(u32 *X, int64_t ncols, u32 sc, const Family *fam, hipStream_t s) {
     constexpr int C = col_tile<LOGM>();
     using TL = Tile<LOGM, C, true>;
     const int64_t nb = (ncols + C - 1) / C;
     ProfScope ps(kProfColNtt, s);
     hipLaunchKernelGGL((k_col_ntt<LOGM, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, ncols,
-                       fam->lv, fam->p, sc);
+                       fam->lv, mod_of(fam), sc);
     return hipGetLastError();
 }
 
-hipError_t launch_col_ntt(int em, u32 *X, int64_t ncols, uint2 sc, const Family *fam,
+hipError_t launch_col_ntt(int em, u32 *X, int64_t ncols, u32 scale, const Family *fam,
                           hipStream_t s) {
     const int k1 = em;
+    const u32 sc = scale_mont(scale, fam->p);
     CM_K1_CASES(col_ntt_t, X, ncols, sc, fam, s)
 }
 

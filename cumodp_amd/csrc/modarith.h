@@ -2,12 +2,17 @@
 //
 // All residues are canonical ([0, p)) at function boundaries.  Three
 // multiplication forms are used:
+//   * Montgomery multiplication with R = 2^32 (P. Montgomery, 1985) in its
+//     lazy form: for a * b < p * 2^32, t = a*b, m = t * (-p^-1) mod 2^32,
+//     (t + m*p) / 2^32 lies in [0, 2p) and equals a*b/R mod p.  On gfx950
+//     this is three full-rate instructions (v_mad_u64_u32 a*b, v_mul_lo_u32,
+//     v_mad_u64_u32 m*p + t).  Transform twiddles and scale constants are
+//     stored in Montgomery form (w R mod p), so one lazy product yields
+//     a * w mod p in [0, 2p) for any 32-bit a.
 //   * Shoup multiplication by a precomputed constant w with companion
 //     wq = floor(w * 2^32 / p): q = mulhi(a, wq), r = a*w - q*p lies in
 //     [0, 2p) for any 32-bit a (V. Shoup, NTL; D. Harvey, "Faster arithmetic
-//     for number-theoretic transforms", 2014).  Three 32-bit multiplies.
-//   * Montgomery multiplication of two variables with R = 2^32:
-//     mont(a, b) = a*b*2^-32 mod p (P. Montgomery, 1985).  Four multiplies.
+//     for number-theoretic transforms", 2014); used off the hot path.
 //   * plain 64-bit remainder on the host for table construction.
 // These replace the double-precision quotient estimate of the reference's
 // mul_mod (include/inlines.h:95-122); outputs are the same canonical residues.
@@ -44,6 +49,20 @@ CM_HD u32 cm_shoup_lazy(u32 a, u32 w, u32 wq, u32 p) {
 
 CM_HD u32 cm_shoup(u32 a, u32 w, u32 wq, u32 p) {
     return cm_reduce2p(cm_shoup_lazy(a, w, wq, p), p);
+}
+
+// Lazy Montgomery product a * b / 2^32 mod p in [0, 2p), for a * b < p 2^32
+// (any 32-bit a when b < p).  npinv = -p^-1 mod 2^32.  p < 2^31 keeps
+// t + m p below 2^64.
+CM_HD u32 cm_mont_lazy(u32 a, u32 b, u32 p, u32 npinv) {
+    const u64 t = (u64)a * b;
+    const u32 m = (u32)t * npinv;
+    return (u32)(((u64)m * p + t) >> 32);
+}
+
+// a * w mod p, canonical, for a twiddle or constant in Montgomery form wm = w R mod p
+CM_HD u32 cm_mul_tw(u32 a, u32 wm, u32 p, u32 npinv) {
+    return cm_reduce2p(cm_mont_lazy(a, wm, p, npinv), p);
 }
 
 // Montgomery: a * b * 2^-32 mod p for a, b < p.
@@ -83,6 +102,12 @@ static inline u32 cm_host_pinv(u32 p) {
 
 // 2^32 mod p
 static inline u32 cm_host_r1(u32 p) { return (u32)(((u64)1 << 32) % p); }
+
+// Montgomery form w 2^32 mod p of w < p
+static inline u32 cm_host_mont(u32 w, u32 p) { return (u32)(((u64)w << 32) % p); }
+
+// -p^-1 mod 2^32
+static inline u32 cm_host_npinv(u32 p) { return 0u - cm_host_pinv(p); }
 
 static inline int cm_fourier_degree(u32 p) {
     u32 q = p - 1;

@@ -22,7 +22,7 @@
 // blocks of 2d words, and maps (x, y) -> (x + y, (x - y) * w_{2d}^(q mod d)).
 //
 // Twiddles come from a level table: for every M = 2, 4, ..., 2^LT the M/2
-// entries w_M^0 .. w_M^(M/2-1) (uint2 {w, floor(w 2^32 / p)}) are stored
+// entries w_M^0 .. w_M^(M/2-1) (Montgomery forms w 2^32 mod p) are stored
 // contiguously at offset M/2 - 1.  Stage s of an L-point transform reads
 // level M = L >> s at index q mod d, so consecutive lanes read consecutive
 // entries (one or a few cache lines per wave instruction).
@@ -38,31 +38,37 @@ CM_DEV int cm_pad(int i) { return i + (i >> 4); }
 
 CM_DEV u32 cm_brev(u32 x, int bits) { return __brev(x) >> (32 - bits); }
 
-// (x, y) -> (x + y, (x - y) w): the difference enters Shoup unreduced.
-CM_DEV void cm_dif_bfly(u32 &x, u32 &y, uint2 w, u32 p) {
-    u32 s = cm_add(x, y, p);
-    u32 d = x - y + p;                 // (0, 2p)
-    y = cm_shoup(d, w.x, w.y, p);
+// Modulus and -p^-1 mod 2^32 for the lazy Montgomery products.
+struct Mod {
+    u32 p, np;
+};
+
+// (x, y) -> (x + y, (x - y) w): the difference enters the product unreduced
+// (x - y + p < 2p, and 2p * w < p 2^32).
+CM_DEV void cm_dif_bfly(u32 &x, u32 &y, u32 w, Mod md) {
+    u32 s = cm_add(x, y, md.p);
+    u32 d = x - y + md.p;              // (0, 2p)
+    y = cm_mul_tw(d, w, md.p, md.np);
     x = s;
 }
 
-CM_DEV void cm_dif_bfly1(u32 &x, u32 &y, u32 p) {
-    u32 s = cm_add(x, y, p);
-    y = cm_sub(x, y, p);
+CM_DEV void cm_dif_bfly1(u32 &x, u32 &y, Mod md) {
+    u32 s = cm_add(x, y, md.p);
+    y = cm_sub(x, y, md.p);
     x = s;
 }
 
 // (a, b) -> (a + b w, a - b w)
-CM_DEV void cm_dit_bfly(u32 &a, u32 &b, uint2 w, u32 p) {
-    u32 t = cm_shoup(b, w.x, w.y, p);
-    b = cm_sub(a, t, p);
-    a = cm_add(a, t, p);
+CM_DEV void cm_dit_bfly(u32 &a, u32 &b, u32 w, Mod md) {
+    u32 t = cm_mul_tw(b, w, md.p, md.np);
+    b = cm_sub(a, t, md.p);
+    a = cm_add(a, t, md.p);
 }
 
-CM_DEV void cm_dit_bfly1(u32 &a, u32 &b, u32 p) {
+CM_DEV void cm_dit_bfly1(u32 &a, u32 &b, Mod md) {
     u32 t = b;
-    b = cm_sub(a, t, p);
-    a = cm_add(a, t, p);
+    b = cm_sub(a, t, md.p);
+    a = cm_add(a, t, md.p);
 }
 
 template <int LOGL_, int C_, bool COLS_, int T_ = (C_ << LOGL_) / 16>
@@ -94,7 +100,7 @@ struct Tile {
 // R DIF stages (S0 .. S0+R-1 of an L = 2^LOGL transform) on NV register
 // groups that share their positions (and so their twiddles).
 template <int LOGL, int R, int S0, int DLOG, int NV>
-CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restrict__ lv, u32 p) {
+CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lv, Mod md) {
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         const int h = 1 << (R - 1 - t);
@@ -105,11 +111,11 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restri
             const int mm = (m & (h - 1)) << DLOG;
             if (DLOG == 0 && mm == 0) {
 #pragma unroll
-                for (int a = 0; a < NV; ++a) cm_dif_bfly1(v[a][m], v[a][m + h], p);
+                for (int a = 0; a < NV; ++a) cm_dif_bfly1(v[a][m], v[a][m + h], md);
             } else {
-                const uint2 w = lv[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+                const u32 w = lv[base + (DLOG == 0 ? 0 : gi_lo) + mm];
 #pragma unroll
-                for (int a = 0; a < NV; ++a) cm_dif_bfly(v[a][m], v[a][m + h], w, p);
+                for (int a = 0; a < NV; ++a) cm_dif_bfly(v[a][m], v[a][m + h], w, md);
             }
         }
     }
@@ -117,7 +123,7 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restri
 
 // Inverse of cm_dif_stages with the inverse level table.
 template <int LOGL, int R, int S0, int DLOG, int NV>
-CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restrict__ lvi, u32 p) {
+CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lvi, Mod md) {
 #pragma unroll
     for (int t = R - 1; t >= 0; --t) {
         const int h = 1 << (R - 1 - t);
@@ -128,11 +134,11 @@ CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const uint2 *__restri
             const int mm = (m & (h - 1)) << DLOG;
             if (DLOG == 0 && mm == 0) {
 #pragma unroll
-                for (int a = 0; a < NV; ++a) cm_dit_bfly1(v[a][m], v[a][m + h], p);
+                for (int a = 0; a < NV; ++a) cm_dit_bfly1(v[a][m], v[a][m + h], md);
             } else {
-                const uint2 w = lvi[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+                const u32 w = lvi[base + (DLOG == 0 ? 0 : gi_lo) + mm];
 #pragma unroll
-                for (int a = 0; a < NV; ++a) cm_dit_bfly(v[a][m], v[a][m + h], w, p);
+                for (int a = 0; a < NV; ++a) cm_dit_bfly(v[a][m], v[a][m + h], w, md);
             }
         }
     }
@@ -174,25 +180,25 @@ struct Round {
     }
     // full forward round on NV tiles, LDS -> LDS
     template <int NV>
-    static CM_DEV void dif(u32 *const (&lds)[NV], const uint2 *lv, u32 p) {
+    static CM_DEV void dif(u32 *const (&lds)[NV], const u32 *lv, Mod md) {
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             G g(threadIdx.x + q * TL::T);
             u32 v[NV][1 << R];
             load<NV>(lds, g, v);
-            cm_dif_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lv, p);
+            cm_dif_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lv, md);
             store<NV>(lds, g, v);
         }
     }
     // full inverse round on NV tiles, LDS -> LDS
     template <int NV>
-    static CM_DEV void dit(u32 *const (&lds)[NV], const uint2 *lvi, u32 p) {
+    static CM_DEV void dit(u32 *const (&lds)[NV], const u32 *lvi, Mod md) {
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             G g(threadIdx.x + q * TL::T);
             u32 v[NV][1 << R];
             load<NV>(lds, g, v);
-            cm_dit_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lvi, p);
+            cm_dit_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lvi, md);
             store<NV>(lds, g, v);
         }
     }
@@ -200,48 +206,47 @@ struct Round {
 
 // Forward rounds [J, J1) on NV tiles with a barrier after each.
 template <class TL, int J, int J1, int NV>
-CM_DEV void cm_dif_rounds(u32 *const (&lds)[NV], const uint2 *lv, u32 p) {
+CM_DEV void cm_dif_rounds(u32 *const (&lds)[NV], const u32 *lv, Mod md) {
     if constexpr (J < J1) {
-        Round<TL, J>::template dif<NV>(lds, lv, p);
+        Round<TL, J>::template dif<NV>(lds, lv, md);
         __syncthreads();
-        cm_dif_rounds<TL, J + 1, J1, NV>(lds, lv, p);
+        cm_dif_rounds<TL, J + 1, J1, NV>(lds, lv, md);
     }
 }
 
 // Inverse rounds J, J-1, ..., J0 (J >= J0) with a barrier after each.
 template <class TL, int J, int J0, int NV>
-CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const uint2 *lvi, u32 p) {
+CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const u32 *lvi, Mod md) {
     if constexpr (J >= J0) {
-        Round<TL, J>::template dit<NV>(lds, lvi, p);
+        Round<TL, J>::template dit<NV>(lds, lvi, md);
         __syncthreads();
-        cm_dit_rounds<TL, J - 1, J0, NV>(lds, lvi, p);
+        cm_dit_rounds<TL, J - 1, J0, NV>(lds, lvi, md);
     }
 }
 
 // Full forward transform of every vector of one tile (bit-reversed result).
 template <class TL>
-CM_DEV void cm_tile_dif(u32 *lds, const uint2 *lv, u32 p) {
+CM_DEV void cm_tile_dif(u32 *lds, const u32 *lv, Mod md) {
     u32 *const t[1] = {lds};
-    cm_dif_rounds<TL, 0, TL::NR, 1>(t, lv, p);
+    cm_dif_rounds<TL, 0, TL::NR, 1>(t, lv, md);
 }
 
 template <class TL>
-CM_DEV void cm_tile_dit(u32 *lds, const uint2 *lvi, u32 p) {
+CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, Mod md) {
     u32 *const t[1] = {lds};
-    cm_dit_rounds<TL, TL::NR - 1, 0, 1>(t, lvi, p);
+    cm_dit_rounds<TL, TL::NR - 1, 0, 1>(t, lvi, md);
 }
 
 // Multiply of two tiles A (first operand, becomes the product) and B:
 // forward transforms of both (shared twiddles), pointwise product scaled by
-// the Shoup constant sc (= 2^32 / N mod p, cancelling Montgomery's 2^-32 and
-// the inverse transform's factor N) fused with the last forward and first
-// inverse round, then the remaining inverse rounds on A.  Barriers included;
-// the product is in A afterwards.
+// sc (Montgomery form of 2^32 / N mod p: the two lazy Montgomery products
+// divide by 2^64, sc restores 2^32 and supplies the inverse transform's 1/N)
+// fused with the last forward and first inverse round, then the remaining
+// inverse rounds on A.  Barriers included; the product is in A afterwards.
 template <class TL>
-CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const uint2 *lv, const uint2 *lvi, u32 p, u32 pinv,
-                             uint2 sc) {
+CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
     u32 *const ab[2] = {A, B};
-    cm_dif_rounds<TL, 0, TL::NR - 1, 2>(ab, lv, p);
+    cm_dif_rounds<TL, 0, TL::NR - 1, 2>(ab, lv, md);
     using RD = Round<TL, TL::NR - 1>;
     constexpr int R = RD::R;
 #pragma unroll
@@ -249,16 +254,16 @@ CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const uint2 *lv, const uint2 *lvi, 
         typename RD::G g(threadIdx.x + q * TL::T);
         u32 v[2][1 << R];
         RD::template load<2>(ab, g, v);
-        cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 2>(v, g.gi_lo, lv, p);
+        cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 2>(v, g.gi_lo, lv, md);
         u32 w[1][1 << R];
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m)
-            w[0][m] = cm_shoup(cm_mont(v[0][m], v[1][m], p, pinv), sc.x, sc.y, p);
-        cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(w, g.gi_lo, lvi, p);
+            w[0][m] = cm_mul_tw(cm_mont_lazy(v[0][m], v[1][m], md.p, md.np), sc, md.p, md.np);
+        cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(w, g.gi_lo, lvi, md);
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) A[g.a(m)] = w[0][m];
     }
     __syncthreads();
     u32 *const a1[1] = {A};
-    cm_dit_rounds<TL, TL::NR - 2, 0, 1>(a1, lvi, p);
+    cm_dit_rounds<TL, TL::NR - 2, 0, 1>(a1, lvi, md);
 }

@@ -28,22 +28,22 @@ std::map<std::tuple<const Family *, int, int>, Outer *> g_outer;
 int g_split[kMaxLog + 1];   // chosen k2 per k (0: default)
 std::map<u32, const Family *> g_default;
 
-uint2 *upload(const std::vector<uint2> &h) {
-    uint2 *d = nullptr;
-    if (hipMalloc((void **)&d, sizeof(uint2) * h.size()) != hipSuccess) return nullptr;
-    if (hipMemcpy(d, h.data(), sizeof(uint2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+u32 *upload(const std::vector<u32> &h) {
+    u32 *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(u32) * h.size()) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(u32) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(d);
         return nullptr;
     }
     return d;
 }
 
-// powers r^0 .. r^(cnt-1) with Shoup companions
-std::vector<uint2> power_table(u32 r, size_t cnt, u32 p) {
-    std::vector<uint2> t(cnt);
-    u64 x = 1;
+// powers r^0 .. r^(cnt-1) in Montgomery form
+std::vector<u32> power_table(u32 r, size_t cnt, u32 p) {
+    std::vector<u32> t(cnt);
+    u64 x = cm_host_mont(1, p);
     for (size_t j = 0; j < cnt; ++j) {
-        t[j] = make_uint2((u32)x, cm_host_shoup_q((u32)x, p));
+        t[j] = (u32)x;
         x = x * r % p;
     }
     return t;
@@ -53,19 +53,20 @@ Family *make_family(u32 p, u32 wN, int K) {
     Family *f = new Family();
     f->p = p;
     f->pinv = cm_host_pinv(p);
+    f->np = cm_host_npinv(p);
     f->K = K;
     f->wN = wN;
     f->LT = K < kMaxRowLog ? K : kMaxRowLog;
     // level tables: level M = 2^j (j = 1 .. LT) at offset 2^(j-1) - 1
-    std::vector<uint2> lv, lvi;
+    std::vector<u32> lv, lvi;
     for (int j = 1; j <= f->LT; ++j) {
         u32 wm = cm_host_pow(wN, (u64)1 << (K - j), p);   // order 2^j
-        std::vector<uint2> a = power_table(wm, (size_t)1 << (j - 1), p);
-        std::vector<uint2> b = power_table(cm_host_inv(wm, p), (size_t)1 << (j - 1), p);
+        std::vector<u32> a = power_table(wm, (size_t)1 << (j - 1), p);
+        std::vector<u32> b = power_table(cm_host_inv(wm, p), (size_t)1 << (j - 1), p);
         lv.insert(lv.end(), a.begin(), a.end());
         lvi.insert(lvi.end(), b.begin(), b.end());
     }
-    if (lv.empty()) { lv.push_back(make_uint2(1, 0)); lvi.push_back(make_uint2(1, 0)); }
+    if (lv.empty()) { lv.push_back(cm_host_mont(1, p)); lvi.push_back(cm_host_mont(1, p)); }
     f->lv = upload(lv);
     f->lvi = upload(lvi);
     if (!f->lv || !f->lvi) {
@@ -151,18 +152,18 @@ const Outer *outer_for(const Family *f, int k) {
     const size_t n1 = (size_t)1 << k1, T = ((size_t)1 << k2) / 16;
     for (int dir = 0; dir < 2; ++dir) {
         const u32 w = dir ? wni : wn;
-        std::vector<uint2> rb(n1 * T), rc(n1 * 16);
+        std::vector<u32> rb(n1 * T), rc(n1 * 16);
         for (size_t r = 0; r < n1; ++r) {
             const u32 wr = cm_host_pow(w, r, p);                 // w^k1
-            u64 x = 1;
+            u64 x = cm_host_mont(1, p);
             for (size_t t = 0; t < T; ++t) {
-                rb[r * T + t] = make_uint2((u32)x, cm_host_shoup_q((u32)x, p));
+                rb[r * T + t] = (u32)x;
                 x = x * wr % p;
             }
             const u32 wrT = cm_host_pow(wr, T, p);              // w^(k1 T)
-            x = 1;
+            x = cm_host_mont(1, p);
             for (size_t m = 0; m < 16; ++m) {
-                rc[r * 16 + m] = make_uint2((u32)x, cm_host_shoup_q((u32)x, p));
+                rc[r * 16 + m] = (u32)x;
                 x = x * wrT % p;
             }
         }

@@ -266,15 +266,14 @@ hipError_t launch_two_convolution(const u64 *a, int64_t na, int wa, const u64 *b
         const Family *fx = family_for_root(p, wx, pl.lk2);
         const Family *fxi = family_for_root(p, cm_host_inv(wx, p), pl.lk2);
         if (!fx || !fxi) return hipErrorOutOfMemory;
-        const uint2 one = make_uint2(1u, 0u);
-        e = launch_col_ntt(pl.lk2, A, Ly, one, fx, s);
-        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, B, Ly, one, fx, s);
+        e = launch_col_ntt(pl.lk2, A, Ly, 1u, fx, s);
+        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, B, Ly, 1u, fx, s);
         // y: 2K independent cyclic products of length Ly (= linear: ny <= Ly)
         if (e == hipSuccess) e = launch_rows_mul(pl.ey, A, B, K2, fam, s);
         // inverse x transform with the 1/(2K) scaling
         const u32 ki = cm_host_inv((u32)K2, p);
         if (e == hipSuccess)
-            e = launch_col_ntt(pl.lk2, A, Ly, make_uint2(ki, cm_host_shoup_q(ki, p)), fxi, s);
+            e = launch_col_ntt(pl.lk2, A, Ly, ki, fxi, s);
         if (e != hipSuccess) return e;
     }
     const u32 p0 = kTcPrimes[0], p1 = kTcPrimes[1], p2 = kTcPrimes[2];
