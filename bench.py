@@ -84,10 +84,12 @@ def timed(fn, reps, sync):
     return float(np.median(ts))
 
 
-def bench_list_multiply(cm, reps=5):
+def bench_list_multiply(cm, cpu, reps=5):
     """4096 pairs of uneven degree (log-uniform in [32, 16384]) mod 469762049:
     one batched cumodp_list_fftmul_dev call against one cumodp_fftmul_uni_dev
-    call per pair, on the same resident inputs."""
+    call per pair, on the same resident inputs.  CPU leg: the oracle's serial
+    NTT multiply over a bounded prefix of the pairs, checked against the
+    batched outputs."""
     w = cm.workload4()
     p = w["p"]
     items, nf, ng, nh = cm.make_items(w["df"], w["dg"])
@@ -99,26 +101,49 @@ def bench_list_multiply(cm, reps=5):
     tb = timed(run_b, reps, cm.synchronize)
     tc = timed(run_c, reps, cm.synchronize)
     H1, H2 = dH1.download(), dH2.download()
-    # spot check three pairs against the oracle's schoolbook product
-    import oracle
-    ok = bool(np.array_equal(H1, H2))
-    for i in (0, len(items) // 2, len(items) - 1):
-        it = items[i]
-        f = w["F"][it.f_off:it.f_off + it.df + 1]
-        g = w["G"][it.g_off:it.g_off + it.dg + 1]
-        ok = ok and bool(np.array_equal(H1[it.h_off:it.h_off + it.df + it.dg + 1],
-                                        oracle.plain_polymul(f, g, p)))
-    for d in (dF, dG, dH1, dH2):
+    # the same batched call captured once into a graph and replayed
+    L = cm.lib()
+    st = L.cumodp_stream_create()
+    dH3 = cm.DeviceArray(nh)
+    g = cm.Graph.capture(st, lambda: cm.list_fftmul_dev(items, dF, dG, dH3, p, st))
+    tg = timed(lambda: g.launch(st), reps, lambda: L.cumodp_stream_synchronize(st))
+    H3 = dH3.download()
+    g.destroy()
+    L.cumodp_stream_destroy(st)
+    for d in (dF, dG, dH1, dH2, dH3):
         d.free()
-    return {"workload": "4096 pairs, degrees log-uniform [32, 16384], mod 469762049",
-            "output_coeffs": nh,
-            "batched": {"ms": tb * 1e3, "coeffs_per_s": nh / tb},
-            "per_call": {"ms": tc * 1e3, "coeffs_per_s": nh / tc},
-            "batched_over_per_call": tc / tb, "check_batched_eq_per_call_eq_oracle": ok}
+    out = {"workload": "4096 pairs, degrees log-uniform [32, 16384], mod 469762049",
+           "output_coeffs": nh,
+           "batched": {"ms": tb * 1e3, "coeffs_per_s": nh / tb},
+           "per_call": {"ms": tc * 1e3, "coeffs_per_s": nh / tc},
+           "graph_replay": {"ms": tg * 1e3, "coeffs_per_s": nh / tg},
+           "batched_over_per_call": tc / tb,
+           "check_batched_eq_per_call": bool(np.array_equal(H1, H2)),
+           "check_graph_eq_batched": bool(np.array_equal(H1, H3)),
+           "tuning": {"list_tile_words": L.cumodp_tune_get_list_tile()}}
+    if cpu:
+        import oracle
+        oracle.lib()
+        t0, k, coeffs, ok = time.perf_counter(), 0, 0, True
+        while k < len(items) and time.perf_counter() - t0 < 8.0:
+            it = items[k]
+            f = w["F"][it.f_off:it.f_off + it.df + 1]
+            g = w["G"][it.g_off:it.g_off + it.dg + 1]
+            h = oracle.ntt_polymul(f, g, p)
+            ok = ok and bool(np.array_equal(H1[it.h_off:it.h_off + it.df + it.dg + 1], h))
+            coeffs += it.df + it.dg + 1
+            k += 1
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": coeffs / t, "unit": "coeffs/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {k} pairs, one serial oracle NTT multiply "
+                                         "per pair", "check_bit_exact": ok}
+    return out
 
 
-def bench_tree(cm, reps=5):
-    """Subproduct tree over 2^16 points mod 469762049, all 17 levels."""
+def bench_tree(cm, cpu, reps=5):
+    """Subproduct tree over 2^16 points mod 469762049, all 17 levels.  CPU
+    leg: the oracle's levels 1..8 (schoolbook per node), compared with the
+    GPU's."""
     w = cm.workload4_tree()
     p, pts = w["p"], w["points"]
     k = 16
@@ -129,12 +154,27 @@ def bench_tree(cm, reps=5):
         dP.addr(), k, p, dT.addr(), None))
     run()
     t = timed(run, reps, cm.synchronize)
-    T = dT.download(count=(1 << k) + 1, offset=L.cumodp_subproduct_tree_offset(k, k))
+    tree = dT.download()
+    root = tree[L.cumodp_subproduct_tree_offset(k, k):]
     r = 123456789
-    ok = eval_mod(T, r, p) == _prod_mod([(r - int(x)) % p for x in pts], p)
+    ok = eval_mod(root, r, p) == _prod_mod([(r - int(x)) % p for x in pts], p)
     dP.free(), dT.free()
-    return {"workload": "subproduct tree, 2^16 points uniform mod 469762049, levels 0..16",
-            "ms": t * 1e3, "tree_words": nw, "check_root_evaluation": bool(ok)}
+    out = {"workload": "subproduct tree, 2^16 points uniform mod 469762049, levels 0..16",
+           "ms": t * 1e3, "tree_words": nw, "check_root_evaluation": bool(ok)}
+    if cpu:
+        import oracle
+        oracle.lib()
+        t0 = time.perf_counter()
+        same = True
+        for lev in range(1, 9):
+            want = oracle.subproduct_level(pts, lev, p).ravel()
+            off = L.cumodp_subproduct_tree_offset(k, lev)
+            same = same and bool(np.array_equal(tree[off:off + want.size], want))
+        tl = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": tl * 1e3, "unit": "ms", "cores": 1, "kind": "port",
+                               "sample": "levels 1..8 of the same tree by the oracle's "
+                                         "schoolbook products", "check_levels_1_8": same}
+    return out
 
 
 def _prod_mod(vals, p):
@@ -144,9 +184,10 @@ def _prod_mod(vals, p):
     return acc
 
 
-def bench_two_convolution(cm, reps=5):
+def bench_two_convolution(cm, cpu, reps=5):
     """Exact product of two degree-16383 int64 polynomials with coefficients
-    uniform in (-10^10, 10^10) (34 bits); output as 2-word integers."""
+    uniform in (-10^10, 10^10) (34 bits); output as 2-word integers.  CPU leg:
+    the oracle's two-convolution restatement and its exact schoolbook."""
     w = cm.workload3()
     a, b = w["a"], w["b"]
     A, B = cm._words(a, 1), cm._words(b, 1)
@@ -162,13 +203,144 @@ def bench_two_convolution(cm, reps=5):
     run()
     t = timed(run, reps, cm.synchronize)
     Cw = dC.download().reshape(nc, wc)
-    import oracle
-    ok = bool(np.array_equal(Cw, oracle.zmul_i64(a, b)[:, :wc]))
     dA.free(), dB.free(), dC.free()
-    return {"workload": "two degree-16383 int64 polynomials, coefficients uniform in "
-                        "(-10^10, 10^10), exact product", "plan": cm.two_convolution_plan(
-                            bits, na, nb),
-            "ms": t * 1e3, "coeffs_per_s": nc / t, "check_exact_vs_schoolbook": ok}
+    out = {"workload": "two degree-16383 int64 polynomials, coefficients uniform in "
+                       "(-10^10, 10^10), exact product",
+           "plan": cm.two_convolution_plan(bits, na, nb),
+           "ms": t * 1e3, "coeffs_per_s": nc / t}
+    if cpu:
+        import oracle
+        oracle.lib()
+        t0 = time.perf_counter()
+        want = oracle.two_convolution_zmul(A, B, wc)
+        t1 = time.perf_counter()
+        school = oracle.zmul_i64(a, b)[:, :wc]
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = {"value": nc / (t1 - t0), "unit": "coeffs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "the full product by the oracle's two-convolution "
+                                         "restatement (3 primes, K=32)",
+                               "schoolbook_s": t2 - t1,
+                               "check_bit_exact": bool(np.array_equal(Cw, want)) and
+                               bool(np.array_equal(Cw, school))}
+    return out
+
+
+def _words_mod(Wd, q):
+    """Two's-complement multi-word integers (rows of uint64 words) mod q."""
+    Wd = np.asarray(Wd, np.uint64)
+    acc = np.zeros(Wd.shape[0], np.uint64)
+    qq = np.uint64(q)
+    for j in range(Wd.shape[1]):
+        f = np.uint64(pow(2, 64 * j, q))
+        lo = Wd[:, j] & np.uint64(0xFFFFFFFF)
+        hi = Wd[:, j] >> np.uint64(32)
+        part = (lo % qq + (hi % qq) * np.uint64(pow(2, 32, q)) % qq) % qq
+        acc = (acc + part * f % qq) % qq
+    neg = (Wd[:, -1] >> np.uint64(63)).astype(bool)
+    top = np.uint64(pow(2, 64 * Wd.shape[1], q))
+    acc[neg] = (acc[neg] + qq - top) % qq
+    return acc
+
+
+def _eval_check(a, b, cw, x, q):
+    """c(x) == a(x) b(x) mod q for int64 a, b and word rows cw."""
+    def ev(v):
+        v = np.asarray(v, np.uint64) % np.uint64(q)
+        n = len(v)
+        pw = np.empty(n, np.uint64)
+        acc = 1
+        blk = min(n, 4096)
+        small = np.array([pow(x, j, q) for j in range(blk)], np.uint64)
+        step = pow(x, blk, q)
+        for s0 in range(0, n, blk):
+            m = min(blk, n - s0)
+            pw[s0:s0 + m] = small[:m] * np.uint64(acc) % np.uint64(q)
+            acc = acc * step % q
+        return int((v * pw % np.uint64(q)).sum() % np.uint64(q))
+    am = np.asarray(a, np.int64) % q
+    bm = np.asarray(b, np.int64) % q
+    return ev(_words_mod(cw, q)) == ev(am) * ev(bm) % q
+
+
+def bench_serving(cm, cpu, nreq=20000):
+    """Exact integer products served as they arrive: 20000 requests, degrees
+    log-uniform in [64, 65536], coefficients of 16/32/48/64 bits (uniform
+    choice), signed uniform.  First all requests arrive at once (capacity);
+    then Poisson arrivals at 80% of that capacity (exponential gaps from the
+    workload stream).  Latency = time from arrival to the product being back
+    in host memory."""
+    from cumodp_amd import serving
+    t0 = time.perf_counter()
+    w = cm.workload5(requests=nreq)
+    Q = serving.make_requests(w)
+    log(f"bench: serving: {Q.n} requests generated in {time.perf_counter() - t0:.1f} s, "
+        f"{Q.total_coeffs} output coefficients")
+    srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30)
+    try:
+        srv.run(Q, np.zeros(Q.n))   # warm-up pass (plans, twiddles, buffers)
+        done0, st0 = srv.run(Q, np.zeros(Q.n))
+        cap_s = float(np.nanmax(done0))
+        cap_rps = Q.n / cap_s
+        lam = 0.8 * cap_rps
+        arr = np.cumsum(w["exp_gaps"][:Q.n]) / lam
+        done1, st1 = srv.run(Q, arr)
+    finally:
+        srv.close()
+    span = float(np.nanmax(done1) - arr[0])
+    lat = serving.latency_summary(done1, arr)
+    # size-independent check on a sample: evaluation at a random point mod
+    # two 31-bit primes
+    idx = np.linspace(0, Q.n - 1, 200).astype(int)
+    ok = all(_eval_check(Q.a(r), Q.b(r), Q.result(r), 987654321, q)
+             for r in idx for q in (2147483647, 2013265921))
+    out = {"workload": f"{Q.n} exact integer products, degrees log-uniform [64, 65536], "
+                       "16/32/48/64-bit signed coefficients, Poisson arrivals at 80% of the "
+                       "measured capacity",
+           "output_coeffs": Q.total_coeffs,
+           "capacity": {"s": cap_s, "coeffs_per_s": Q.total_coeffs / cap_s,
+                        "requests_per_s": cap_rps, **st0},
+           "poisson": {"rate_requests_per_s": lam, "coeffs_per_s": Q.total_coeffs / span,
+                       **lat, **st1},
+           "check_evaluation_sample": bool(ok)}
+    if cpu:
+        import oracle
+        oracle.lib()
+        t0, k, coeffs, same, alone = time.perf_counter(), 0, 0, True, True
+        order = np.argsort(Q.nc)[::max(1, Q.n // 64)]
+        for r in order:
+            if time.perf_counter() - t0 > 10.0:
+                break
+            A = Q.a(r).view(np.uint64).reshape(-1, 1)
+            B = Q.b(r).view(np.uint64).reshape(-1, 1)
+            want = oracle.two_convolution_zmul(A, B, int(Q.wc[r]))
+            same = same and bool(np.array_equal(Q.result(r), want))
+            coeffs += int(Q.nc[r])
+            k += 1
+        t = time.perf_counter() - t0
+        for r in order[:8]:
+            one = cm.two_convolution_poly_mul(Q.a(r), Q.b(r), int(Q.wc[r]))
+            alone = alone and bool(np.array_equal(one, Q.result(r)))
+        out["cpu_baseline"] = {"value": coeffs / t, "unit": "coeffs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{k} requests spread over the size range, the "
+                                         "oracle's two-convolution restatement each",
+                               "check_bit_exact": same, "check_alone_eq_in_mix": alone}
+    Q.free()
+    return out
+
+
+def copy_calibration(nbytes=256 << 20):
+    """A plain device-to-device copy kernel of known size: the FETCH_SIZE and
+    WRITE_SIZE counters of its dispatches calibrate the counters of the
+    multiply kernels (bytes counted per byte moved)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"bench: copied {nbytes} bytes three times")
 
 
 def main():
@@ -179,7 +351,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--primary-only", action="store_true",
                     help="skip the list-multiply, tree and integer-multiply lines")
+    ap.add_argument("--autotune", action="store_true",
+                    help="measure and apply the tunables (column tile, four-step split, "
+                         "list tile) before the timed runs")
+    ap.add_argument("--serving-requests", type=int, default=20000,
+                    help="requests of the serving line (0 skips it)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (reference for "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path is single-GPU); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -187,6 +370,10 @@ def main():
 
     import cumodp_amd as cm
     L = cm.lib()
+    tuning = None
+    if args.autotune:
+        from cumodp_amd import autotune
+        tuning = autotune.tune_all(log=log)
     w = cm.workload2()
     F, G, p = w["F"], w["G"], w["p"]
     n = 1 << 24
@@ -260,6 +447,8 @@ def main():
                      "avg_launch_us": avg_s * 1e6},
         "whole_multiply_hbm_frac_48n": whole / HBM_PEAK_GBS,
         "kernels": kernels,
+        "tuning": {"autotuned": bool(args.autotune), "split_k2": L.cumodp_tune_get_split(24),
+                   "col_tile_words": L.cumodp_tune_get_col_tile()},
         "check": {"evaluation_identity": ok_eval},
     }
     if not args.no_cpu_baseline:
@@ -275,11 +464,16 @@ def main():
     dF.free(), dG.free(), dH.free()
     if not args.primary_only:
         sec = {}
+        cpu = not args.no_cpu_baseline
         for key, fn in (("list_multiply", bench_list_multiply), ("subproduct_tree", bench_tree),
                         ("two_convolution", bench_two_convolution)):
             log(f"bench: {key}")
-            sec[key] = fn(cm)
+            sec[key] = fn(cm, cpu)
             log(f"bench: {key}: {sec[key]}")
+        if args.serving_requests > 0:
+            log("bench: serving")
+            sec["serving"] = bench_serving(cm, cpu, args.serving_requests)
+            log(f"bench: serving: {sec['serving']}")
         out["secondary"] = sec
     print(json.dumps(out), flush=True)
 

@@ -78,6 +78,13 @@ class MulItem(C.Structure):
                 ("df", C.c_int32), ("dg", C.c_int32)]
 
 
+class ZmulItem(C.Structure):
+    """cumodp_zmul_item of include/cumodp_amd.h (word offsets and widths)"""
+    _fields_ = [("a_off", C.c_int64), ("b_off", C.c_int64), ("c_off", C.c_int64),
+                ("na", C.c_int64), ("nb", C.c_int64), ("wa", C.c_int32), ("wb", C.c_int32),
+                ("wc", C.c_int32), ("bits", C.c_int32)]
+
+
 def lib():
     """Load libcumodp_amd.so (raises if it has not been built)."""
     global _LIB
@@ -132,6 +139,14 @@ def lib():
         L.cumodp_profile_read.restype = err
         L.cumodp_tune_set_split.argtypes = [s, s]
         L.cumodp_tune_set_split.restype = err
+        L.cumodp_tune_get_split.argtypes = [s]
+        L.cumodp_tune_get_split.restype = s
+        L.cumodp_tune_set_list_tile.argtypes = [s]
+        L.cumodp_tune_set_list_tile.restype = err
+        L.cumodp_tune_get_list_tile.restype = s
+        L.cumodp_tune_set_col_tile.argtypes = [s]
+        L.cumodp_tune_set_col_tile.restype = err
+        L.cumodp_tune_get_col_tile.restype = s
         i64 = C.c_int64
         L.cumodp_two_convolution_poly_mul.argtypes = [vp, i64, s, vp, i64, s, vp, s]
         L.cumodp_two_convolution_poly_mul.restype = err
@@ -154,6 +169,25 @@ def lib():
         L.cumodp_subproduct_tree_dev.restype = err
         L.cumodp_subproduct_tree.argtypes = [vp, s, s, vp]
         L.cumodp_subproduct_tree.restype = err
+        L.cumodp_two_convolution_list_dev.argtypes = [C.POINTER(ZmulItem), i64, vp, vp, vp, vp]
+        L.cumodp_two_convolution_list_dev.restype = err
+        L.cumodp_host_alloc.argtypes = [C.c_size_t]
+        L.cumodp_host_alloc.restype = vp
+        L.cumodp_host_free.argtypes = [vp]
+        L.cumodp_host_free.restype = err
+        for n in ("cumodp_memcpy_h2d_async", "cumodp_memcpy_d2h_async"):
+            getattr(L, n).argtypes = [vp, vp, C.c_size_t, vp]
+            getattr(L, n).restype = err
+        L.cumodp_stream_query.argtypes = [vp]
+        L.cumodp_stream_query.restype = s
+        L.cumodp_graph_begin.argtypes = [vp]
+        L.cumodp_graph_begin.restype = err
+        L.cumodp_graph_end.argtypes = [vp]
+        L.cumodp_graph_end.restype = vp
+        L.cumodp_graph_launch.argtypes = [vp, vp]
+        L.cumodp_graph_launch.restype = err
+        L.cumodp_graph_destroy.argtypes = [vp]
+        L.cumodp_graph_destroy.restype = err
         _LIB = L
     return _LIB
 
@@ -367,3 +401,47 @@ def subproduct_tree(points, p):
         off = L.cumodp_subproduct_tree_offset(k, i)
         out.append(tree[off:off + (1 << k)  + (1 << (k - i))].reshape(1 << (k - i), (1 << i) + 1))
     return out
+
+
+# ---- graphs of repeated shapes -------------------------------------------------
+class Graph:
+    """A captured sequence of stream-ordered calls, replayed with one launch.
+
+        g = Graph.capture(stream, fn)   # fn() issues cumodp_*_dev calls on stream
+        g.launch(stream)
+
+    fn runs once eagerly first (scratch and twiddle tables are allocated
+    then), and again under capture."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def capture(cls, stream, fn, warm=True):
+        L = lib()
+        if warm:
+            fn()
+            _check("cumodp_stream_synchronize", L.cumodp_stream_synchronize(C.c_void_p(stream)))
+        _check("cumodp_graph_begin", L.cumodp_graph_begin(C.c_void_p(stream)))
+        try:
+            fn()
+        finally:
+            h = L.cumodp_graph_end(C.c_void_p(stream))
+        if not h:
+            raise CumodpError("cumodp_graph_end", CUMODP_FAILURE)
+        return cls(h)
+
+    def launch(self, stream):
+        _check("cumodp_graph_launch", lib().cumodp_graph_launch(C.c_void_p(self.handle),
+                                                                C.c_void_p(stream)))
+
+    def destroy(self):
+        if self.handle:
+            lib().cumodp_graph_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

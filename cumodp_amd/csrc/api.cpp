@@ -68,8 +68,8 @@ cumodp_err mul_dev_impl(u32 *Hd, int64_t nout, const u32 *Fd, int64_t lf, const 
     } else {
         // the column kernels read the operands with zero padding and write
         // the first nout product words straight to Hd
-        u32 *wf = (u32 *)workspace(0, sizeof(u32) * (size_t)n);
-        u32 *wg = (u32 *)workspace(1, sizeof(u32) * (size_t)n);
+        u32 *wf = (u32 *)workspace(0, sizeof(u32) * (size_t)n, s);
+        u32 *wg = (u32 *)workspace(1, sizeof(u32) * (size_t)n, s);
         if (!wf || !wg) return fail(CUMODP_OUT_OF_MEMORY, "multiply workspace");
         const int64_t nw = nout < n ? nout : n;
         FourStepIO io{Fd, Gd, lf, lg, 0, wf, wg, Hd, nw, 0, 1};
@@ -90,7 +90,7 @@ cumodp_err mul_host_impl(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg,
     if (ce != CUMODP_SUCCESS) return ce;
     std::lock_guard<std::mutex> lk(api_mutex());
     const int64_t nout = (int64_t)dh + 1 < n ? (int64_t)dh + 1 : n;
-    u32 *buf = (u32 *)workspace(2, sizeof(u32) * (size_t)(df + 1 + dg + 1 + nout));
+    u32 *buf = (u32 *)workspace(2, sizeof(u32) * (size_t)(df + 1 + dg + 1 + nout), 0);
     if (!buf) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
     u32 *dF = buf, *dG = buf + df + 1, *dH = dG + dg + 1;
     hipStream_t st = 0;
@@ -121,7 +121,7 @@ cumodp_err ntt_dev_impl(u32 *X, int64_t m, int k, u32 w, u32 p, bool inverse, hi
     } else {
         const int64_t n = (int64_t)1 << k;
         if (m > INT_MAX) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "too many rows");
-        u32 *Y = (u32 *)workspace(3, sizeof(u32) * (size_t)(n * m));
+        u32 *Y = (u32 *)workspace(3, sizeof(u32) * (size_t)(n * m), s);
         if (!Y) return fail(CUMODP_OUT_OF_MEMORY, "transform workspace");
         he = launch_ntt_four_step(k, X, Y, (int)m, inverse, fam, s);
     }
@@ -134,7 +134,7 @@ cumodp_err ntt_host_impl(sfixn *X, int64_t m, int k, u32 w, u32 p, bool inverse)
     if (k < 4) return fail(CUMODP_FFT_SIZE_TOO_SMALL, "transform size below 16");
     std::lock_guard<std::mutex> lk(api_mutex());
     const size_t bytes = sizeof(u32) * (size_t)(m << k);
-    u32 *d = (u32 *)workspace(4, bytes);
+    u32 *d = (u32 *)workspace(4, bytes, 0);
     if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
     hipError_t he = hipMemcpy(d, X, bytes, hipMemcpyHostToDevice);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
@@ -169,7 +169,7 @@ cumodp_err biv_host_impl(sfixn *X, int em, u32 wm, int en, u32 wn, u32 p, bool i
     if (em < 1 || en < 4 || em + en > 30) return fail(CUMODP_ASSUMPTION_ERROR, "bivariate sizes");
     std::lock_guard<std::mutex> lk(api_mutex());
     const size_t bytes = sizeof(u32) << (em + en);
-    u32 *d = (u32 *)workspace(4, bytes);
+    u32 *d = (u32 *)workspace(4, bytes, 0);
     if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
     hipError_t he = hipMemcpy(d, X, bytes, hipMemcpyHostToDevice);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
@@ -326,8 +326,19 @@ void *cumodp_stream_create(void) {
 }
 
 cumodp_err cumodp_stream_destroy(void *stream) {
-    hipError_t e = hipStreamDestroy(as_stream(stream));
+    hipError_t e = hipStreamSynchronize(as_stream(stream));
+    if (e != hipSuccess) return fail(CUMODP_FAILURE, "stream destroy", e);
+    release_workspace(as_stream(stream));
+    e = hipStreamDestroy(as_stream(stream));
     return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "stream destroy", e);
+}
+
+int cumodp_stream_query(void *stream) {
+    hipError_t e = hipStreamQuery(as_stream(stream));
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    fail(CUMODP_FAILURE, "stream query", e);
+    return -1;
 }
 
 cumodp_err cumodp_stream_synchronize(void *stream) {
@@ -398,7 +409,7 @@ cumodp_err cumodp_two_convolution_poly_mul(const uint64_t *a, int64_t na, int wa
         return fail(CUMODP_ASSUMPTION_ERROR, "output words too narrow for the product");
     const int64_t nc = na + nb - 1;
     std::lock_guard<std::mutex> lk(api_mutex());
-    u64 *d = (u64 *)workspace(12, sizeof(u64) * (size_t)(na * wa + nb * wb + nc * wc));
+    u64 *d = (u64 *)workspace(12, sizeof(u64) * (size_t)(na * wa + nb * wb + nc * wc), 0);
     if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
     u64 *da = d, *db = da + na * wa, *dc = db + nb * wb;
     hipError_t he = hipMemcpy(da, a, sizeof(u64) * (size_t)(na * wa), hipMemcpyHostToDevice);
@@ -409,6 +420,52 @@ cumodp_err cumodp_two_convolution_poly_mul(const uint64_t *a, int64_t na, int wa
     he = hipMemcpy(c, dc, sizeof(u64) * (size_t)(nc * wc), hipMemcpyDeviceToHost);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
     return CUMODP_SUCCESS;
+}
+
+cumodp_err cumodp_two_convolution_list_dev(const cumodp_zmul_item *items, int64_t count,
+                                           const uint64_t *A, const uint64_t *B, uint64_t *C,
+                                           void *stream) {
+    static_assert(sizeof(cumodp_zmul_item) == sizeof(TcItem), "item layout");
+    if (count < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative count");
+    if (count == 0) return CUMODP_SUCCESS;
+    for (int64_t i = 0; i < count; ++i) {
+        const cumodp_zmul_item &it = items[i];
+        if (it.na < 1 || it.nb < 1 || it.wa < 1 || it.wb < 1 || it.wc < 1 || it.bits < 1 ||
+            it.a_off < 0 || it.b_off < 0 || it.c_off < 0)
+            return fail(CUMODP_ASSUMPTION_ERROR, "bad integer product item");
+        if (it.bits > 64 * std::max(it.wa, it.wb))
+            return fail(CUMODP_ASSUMPTION_ERROR, "bits exceed the operand words");
+        if (64 * it.wc < tc_out_bits(it.bits, it.bits, it.na, it.nb))
+            return fail(CUMODP_ASSUMPTION_ERROR, "output words too narrow for the product");
+    }
+    std::lock_guard<std::mutex> lk(api_mutex());
+    int err = 0;
+    hipError_t he = launch_two_convolution_list((const TcItem *)items, count, (const u64 *)A,
+                                                (const u64 *)B, (u64 *)C, as_stream(stream), &err);
+    if (err) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "no two-convolution base fits an item");
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "two-convolution list", he);
+    return CUMODP_SUCCESS;
+}
+
+void *cumodp_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+cumodp_err cumodp_host_free(void *ptr) {
+    hipError_t e = hipHostFree(ptr);
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "host free", e);
+}
+
+cumodp_err cumodp_memcpy_h2d_async(void *dst, const void *src, size_t bytes, void *stream) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream));
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "copy to device", e);
+}
+
+cumodp_err cumodp_memcpy_d2h_async(void *dst, const void *src, size_t bytes, void *stream) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream));
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "copy to host", e);
 }
 
 int cumodp_find_largest_bit_width(const uint64_t *a, int64_t na, int wa, const uint64_t *b,
@@ -495,7 +552,7 @@ cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p, sfixn *tr
     if (ce != CUMODP_SUCCESS) return ce;
     const int64_t n = (int64_t)1 << k, words = tree_level_offset(k, k + 1);
     std::lock_guard<std::mutex> lk(api_mutex());
-    u32 *d = (u32 *)workspace(13, sizeof(u32) * (size_t)(n + words));
+    u32 *d = (u32 *)workspace(13, sizeof(u32) * (size_t)(n + words), 0);
     if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
     const Family *fam = family_default((u32)p);
     if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
@@ -506,6 +563,85 @@ cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p, sfixn *tr
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "subproduct tree", he);
     return CUMODP_SUCCESS;
 }
+
+namespace {
+struct CapturedGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> host_bufs;   // pinned descriptor copies the replays read
+};
+}  // namespace
+
+cumodp_err cumodp_graph_begin(void *stream) {
+    if (!stream) return fail(CUMODP_ASSUMPTION_ERROR, "graphs need a created stream");
+    hipError_t e = hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeRelaxed);
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "begin capture", e);
+}
+
+void *cumodp_graph_end(void *stream) {
+    if (!stream) {
+        fail(CUMODP_ASSUMPTION_ERROR, "graphs need a created stream");
+        return nullptr;
+    }
+    hipStream_t s = as_stream(stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    std::vector<void *> bufs = take_captured_buffers(s);
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        fail(CUMODP_FAILURE, "end capture", e);
+        if (g) (void)hipGraphDestroy(g);
+        for (void *h : bufs) (void)hipHostFree(h);
+        return nullptr;
+    }
+    CapturedGraph *cg = new CapturedGraph();
+    cg->graph = g;
+    cg->exec = x;
+    cg->host_bufs.swap(bufs);
+    return cg;
+}
+
+cumodp_err cumodp_graph_launch(void *graph, void *stream) {
+    if (!graph) return fail(CUMODP_ASSUMPTION_ERROR, "null graph");
+    hipError_t e = hipGraphLaunch(((CapturedGraph *)graph)->exec, as_stream(stream));
+    return e == hipSuccess ? CUMODP_SUCCESS : fail(CUMODP_FAILURE, "graph launch", e);
+}
+
+cumodp_err cumodp_graph_destroy(void *graph) {
+    if (!graph) return CUMODP_SUCCESS;
+    CapturedGraph *cg = (CapturedGraph *)graph;
+    // replays still queued may read the graph's descriptor copies
+    (void)hipDeviceSynchronize();
+    if (cg->exec) (void)hipGraphExecDestroy(cg->exec);
+    if (cg->graph) (void)hipGraphDestroy(cg->graph);
+    for (void *h : cg->host_bufs) (void)hipHostFree(h);
+    delete cg;
+    return CUMODP_SUCCESS;
+}
+
+int cumodp_tune_get_split(int k) {
+    if (k <= kMaxLdsLog || k > kMaxLog) return 0;
+    int k1, k2;
+    four_step_split(k, k1, k2);
+    return k2;
+}
+
+cumodp_err cumodp_tune_set_list_tile(int words) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return set_list_tile_words(words) ? CUMODP_SUCCESS
+                                      : fail(CUMODP_ASSUMPTION_ERROR, "tile words: 2048, 4096 or 8192");
+}
+
+int cumodp_tune_get_list_tile(void) { return list_tile_words(); }
+
+cumodp_err cumodp_tune_set_col_tile(int words) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return set_col_tile_words(words) ? CUMODP_SUCCESS
+                                     : fail(CUMODP_ASSUMPTION_ERROR, "column tile words: 8192 or 16384");
+}
+
+int cumodp_tune_get_col_tile(void) { return col_tile_words(); }
 
 cumodp_err cumodp_tune_set_split(int k, int k2) {
     return set_four_step_split(k, k2) ? CUMODP_SUCCESS

@@ -4,11 +4,12 @@
 // (src/list_stockham.cu:595-630) transforms m rows of one length, and the
 // unbuilt draft src/.list_fft_poly_mul.cu:15-53 multiplies equal-length
 // pairs.  Here products are binned by transform size 2^e:
-//   * e <= 12: one fused launch for every such product.  Each 256-thread
-//     workgroup takes a 4096-word tile holding 4096 / 2^e products of one
-//     size and runs the whole multiply in LDS; the tile's size selects the
-//     code path at run time, so one launch covers all sizes;
-//   * e == 13: one launch of the 8192-point workgroup multiply;
+//   * 2^e <= W (W = 2048, 4096 or 8192 words, a tuned choice): one fused
+//     launch for every such product.  Each workgroup takes a W-word tile
+//     holding W / 2^e products of one size and runs the whole multiply in
+//     LDS; the tile's size selects the code path at run time, so one launch
+//     covers all sizes;
+//   * 2^e <= 2^13 above W: one launch of the 8192-point workgroup multiply;
 //   * e >= 14: per size, the inputs are gathered into zero-padded slabs,
 //     multiplied by the batched four-step kernels and scattered back.
 #include <algorithm>
@@ -27,21 +28,27 @@ struct TileDesc {
     int32_t pad;
 };
 
-constexpr int kMixT = 256;          // threads of the mixed-size workgroup
-constexpr int kMixWords = 4096;     // words per tile
+// Mixed-size tile variants (words per tile, threads per workgroup); the
+// tuner picks one per process (cumodp_tune_set_list_tile).  Products of
+// 2^logn <= words points go to the mixed kernel, words / 2^logn per tile.
+constexpr int kMixVariants = 3;
+constexpr int kMixWordsOf[kMixVariants] = {2048, 4096, 8192};
+constexpr int kMixThreadsOf[kMixVariants] = {128, 256, 512};
+int g_mix_variant = 1;   // 4096 words, 256 threads: measured default
 
 struct Scales {
     u32 v[16];
 };
 
-template <int LOGL>
-using MixTile = Tile<LOGL, (kMixWords >> LOGL), false, kMixT>;
+template <int LOGL, int W, int T>
+using MixTile = Tile<LOGL, (W >> LOGL), false, T>;
 
-template <int LOGL>
+template <int LOGL, int W, int T>
 CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const MulDesc *d,
                      int cnt, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
-    using TL = MixTile<LOGL>;
+    using TL = MixTile<LOGL, W, T>;
     constexpr int L = TL::L;
+    constexpr int kMixT = T;
     // element idx of the tile: vector c = idx >> LOGL, position i; all loads
     // of a thread are issued before the first LDS store
     constexpr int PER = TL::WORDS / kMixT;
@@ -71,28 +78,28 @@ CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const M
     }
 }
 
-__global__ __launch_bounds__(kMixT) void k_mul_mixed(
+template <int W, int T>
+__global__ __launch_bounds__(T) void k_mul_mixed(
     const u32 *__restrict__ F, const u32 *__restrict__ G, u32 *__restrict__ H,
     const MulDesc *__restrict__ descs, const TileDesc *__restrict__ tiles,
     const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md, Scales scales) {
-    constexpr int W = kMixWords + (kMixWords >> 4);
-    __shared__ u32 lds[2 * W];
-    u32 *A = lds, *B = lds + W;
+    constexpr int LW = W + (W >> 4);
+    __shared__ u32 lds[2 * LW];
+    u32 *A = lds, *B = lds + LW;
     const TileDesc t = tiles[blockIdx.x];
     const MulDesc *d = descs + t.first;
     const u32 sc = scales.v[t.logn];
+#define CM_MIX(K)                                                                        \
+    case K:                                                                              \
+        if constexpr ((1 << K) <= W && (W >> K) * (1 << K) / T >= 16)                    \
+            mul_tile<K, W, T>(A, B, F, G, H, d, t.count, lv, lvi, md, sc);               \
+        break;
     switch (t.logn) {
-    case 4: mul_tile<4>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 5: mul_tile<5>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 6: mul_tile<6>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 7: mul_tile<7>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 8: mul_tile<8>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 9: mul_tile<9>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 10: mul_tile<10>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 11: mul_tile<11>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
-    case 12: mul_tile<12>(A, B, F, G, H, d, t.count, lv, lvi, md, sc); break;
+        CM_MIX(4) CM_MIX(5) CM_MIX(6) CM_MIX(7) CM_MIX(8) CM_MIX(9) CM_MIX(10) CM_MIX(11)
+        CM_MIX(12) CM_MIX(13)
     default: break;
     }
+#undef CM_MIX
 }
 
 // zero-padded copy of operand `which` (0: F, 1: G) of descriptor row y into
@@ -119,42 +126,6 @@ __global__ void k_scatter(u32 *__restrict__ H, const u32 *__restrict__ slab, int
 
 namespace {
 
-// Pinned staging for descriptors; reused once the previous upload finished.
-struct Staging {
-    void *host = nullptr;
-    size_t cap = 0;
-    hipEvent_t done = nullptr;
-};
-
-Staging &staging() {
-    static Staging s;
-    return s;
-}
-
-hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s) {
-    Staging &st = staging();
-    hipError_t e;
-    if (!st.done) {
-        e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
-        if (e != hipSuccess) return e;
-    } else {
-        e = hipEventSynchronize(st.done);
-        if (e != hipSuccess) return e;
-    }
-    if (st.cap < bytes) {
-        if (st.host) (void)hipHostFree(st.host);
-        st.host = nullptr;
-        st.cap = 0;
-        e = hipHostMalloc(&st.host, bytes);
-        if (e != hipSuccess) return e;
-        st.cap = bytes;
-    }
-    memcpy(st.host, src, bytes);
-    e = hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    return hipEventRecord(st.done, s);
-}
-
 // pointwise constants of the 2^e-point products, e = 0 .. 15
 Scales scale_table(const Family *fam) {
     Scales t;
@@ -163,6 +134,17 @@ Scales scale_table(const Family *fam) {
 }
 
 }  // namespace
+
+bool set_list_tile_words(int words) {
+    for (int v = 0; v < kMixVariants; ++v)
+        if (kMixWordsOf[v] == words) {
+            g_mix_variant = v;
+            return true;
+        }
+    return false;
+}
+
+int list_tile_words() { return kMixWordsOf[g_mix_variant]; }
 
 hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, const u32 *G,
                            u32 *H, const Family *fam, hipStream_t s) {
@@ -174,26 +156,28 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
                      [&](int64_t a, int64_t b) { return descs[a].logn < descs[b].logn; });
     std::vector<MulDesc> sorted((size_t)count);
     for (int64_t i = 0; i < count; ++i) sorted[(size_t)i] = descs[order[(size_t)i]];
+    const int mv = g_mix_variant;
+    const int mix_words = kMixWordsOf[mv];
     std::vector<TileDesc> tiles;
     int64_t i = 0;
-    while (i < count && sorted[(size_t)i].logn <= 12) {
+    while (i < count && ((int64_t)1 << sorted[(size_t)i].logn) <= mix_words) {
         const int lg = sorted[(size_t)i].logn;
-        const int per = kMixWords >> lg;
+        const int per = mix_words >> lg;
         int64_t j = i;
         while (j < count && sorted[(size_t)j].logn == lg && j - i < per) ++j;
         tiles.push_back(TileDesc{(int32_t)i, (int32_t)(j - i), lg, 0});
         i = j;
     }
     const int64_t n_small_end = i;
-    int64_t n13_end = n_small_end;
-    while (n13_end < count && sorted[(size_t)n13_end].logn == 13) ++n13_end;
+    int64_t n13_end = n_small_end;   // end of the one-product-per-workgroup sizes
+    while (n13_end < count && sorted[(size_t)n13_end].logn <= kMaxLdsLog) ++n13_end;
 
     const size_t desc_bytes = sizeof(MulDesc) * (size_t)count;
     const size_t tile_bytes = sizeof(TileDesc) * tiles.size();
     std::vector<char> blob(desc_bytes + tile_bytes);
     memcpy(blob.data(), sorted.data(), desc_bytes);
     if (tile_bytes) memcpy(blob.data() + desc_bytes, tiles.data(), tile_bytes);
-    char *dblob = (char *)workspace(7, blob.size());
+    char *dblob = (char *)workspace(7, blob.size(), s);
     if (!dblob) return hipErrorOutOfMemory;
     hipError_t e = stage_upload(blob.data(), blob.size(), dblob, s);
     if (e != hipSuccess) return e;
@@ -202,16 +186,28 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
 
     if (!tiles.empty()) {
         ProfScope ps(kProfMulMixed, s);
-        hipLaunchKernelGGL(k_mul_mixed, dim3((unsigned)tiles.size()), dim3(kMixT), 0, s, F, G, H,
-                           d_descs, d_tiles, fam->lv, fam->lvi, Mod{fam->p, fam->np},
-                           scale_table(fam));
+        const Mod md{fam->p, fam->np};
+        const Scales sc = scale_table(fam);
+        const dim3 g((unsigned)tiles.size());
+        if (mv == 0)
+            hipLaunchKernelGGL((k_mul_mixed<2048, 128>), g, dim3(128), 0, s, F, G, H, d_descs,
+                               d_tiles, fam->lv, fam->lvi, md, sc);
+        else if (mv == 1)
+            hipLaunchKernelGGL((k_mul_mixed<4096, 256>), g, dim3(256), 0, s, F, G, H, d_descs,
+                               d_tiles, fam->lv, fam->lvi, md, sc);
+        else
+            hipLaunchKernelGGL((k_mul_mixed<8192, 512>), g, dim3(512), 0, s, F, G, H, d_descs,
+                               d_tiles, fam->lv, fam->lvi, md, sc);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (n13_end > n_small_end) {
-        e = launch_mul_lds(13, F, G, H, d_descs + n_small_end, (int)(n13_end - n_small_end), fam,
-                           s);
+    for (int64_t a = n_small_end; a < n13_end;) {
+        const int lg = sorted[(size_t)a].logn;
+        int64_t b = a;
+        while (b < n13_end && sorted[(size_t)b].logn == lg) ++b;
+        e = launch_mul_lds(lg, F, G, H, d_descs + a, (int)(b - a), fam, s);
         if (e != hipSuccess) return e;
+        a = b;
     }
     for (int64_t a = n13_end; a < count;) {
         const int lg = sorted[(size_t)a].logn;
@@ -222,8 +218,8 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
         const int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 28) / n);
         for (int64_t c0 = a; c0 < b; c0 += chunk) {
             const int cnt = (int)std::min<int64_t>(chunk, b - c0);
-            u32 *sf = (u32 *)workspace(5, sizeof(u32) * (size_t)(n * cnt));
-            u32 *sg = (u32 *)workspace(6, sizeof(u32) * (size_t)(n * cnt));
+            u32 *sf = (u32 *)workspace(5, sizeof(u32) * (size_t)(n * cnt), s);
+            u32 *sg = (u32 *)workspace(6, sizeof(u32) * (size_t)(n * cnt), s);
             if (!sf || !sg) return hipErrorOutOfMemory;
             const unsigned gx = (unsigned)std::min<int64_t>(n / 1024, 64);
             {

@@ -436,17 +436,31 @@ hipError_t launch_ntt_lds(int k, u32 *X, int64_t m, bool inverse, u32 scale, con
     return hipErrorInvalidValue;
 }
 
-// column tile width: C columns x 2^LOGL1 rows <= 16384 words
-template <int LOGL1>
+// column tile width: C columns x 2^LOGL1 rows <= WORDS words (at most 4096
+// columns)
+template <int LOGL1, int WORDS = 16384>
 static constexpr int col_tile() {
-    return (16384 >> LOGL1) > 4096 ? 4096 : (16384 >> LOGL1);
+    return (WORDS >> LOGL1) > 4096 ? 4096 : (WORDS >> LOGL1);
 }
 
-template <int LOGL1>
-static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
+// Words per column tile of the multiply's four-step column passes: 16384
+// (two 1024-thread workgroups per CU) or 8192 (four 512-thread workgroups,
+// half the row segment per tile); a tuned choice.
+static int g_col_words = 16384;
+
+bool set_col_tile_words(int words) {
+    if (words != 8192 && words != 16384) return false;
+    g_col_words = words;
+    return true;
+}
+
+int col_tile_words() { return g_col_words; }
+
+template <int LOGL1, int WORDS>
+static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
                             const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
                             hipStream_t s) {
-    constexpr int C = col_tile<LOGL1>();
+    constexpr int C = col_tile<LOGL1, WORDS>();
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColFwd, s);
@@ -460,15 +474,33 @@ static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cn
 }
 
 template <int LOGL1>
-static hipError_t col_inv_t(const ColSet &S, int cnt, int logn2, const Family *fam,
+static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
+                            const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
                             hipStream_t s) {
-    constexpr int C = col_tile<LOGL1>();
+    // the 8192-word tile serves the multiply only (stand-alone transforms
+    // pass outer twiddles)
+    if (!olo && g_col_words == 8192 && LOGL1 >= 9)
+        return col_fwd_w<LOGL1, 8192>(s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, md, s);
+    return col_fwd_w<LOGL1, 16384>(s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, md, s);
+}
+
+template <int LOGL1, int WORDS>
+static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *fam,
+                            hipStream_t s) {
+    constexpr int C = col_tile<LOGL1, WORDS>();
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColInv, s);
     hipLaunchKernelGGL((k_col_inv<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S, logn2,
                        fam->lvi, mod_of(fam));
     return hipGetLastError();
+}
+
+template <int LOGL1>
+static hipError_t col_inv_t(const ColSet &S, int cnt, int logn2, const Family *fam,
+                            hipStream_t s) {
+    if (g_col_words == 8192 && LOGL1 >= 9) return col_inv_w<LOGL1, 8192>(S, cnt, logn2, fam, s);
+    return col_inv_w<LOGL1, 16384>(S, cnt, logn2, fam, s);
 }
 
 template <int LOGL2>

@@ -31,7 +31,13 @@ CM_HD u32 cm_reduce2p(u32 r, u32 p) { u32 t = r - p; return t < r ? t : r; }
 
 CM_HD u32 cm_add(u32 a, u32 b, u32 p) { return cm_reduce2p(a + b, p); }
 
-CM_HD u32 cm_sub(u32 a, u32 b, u32 p) { return cm_reduce2p(a - b + p, p); }
+// a - b for canonical a, b: when a < b the difference wraps above 2^31 and
+// adding p wraps it back into [0, p), so the smaller of d and d + p is the
+// residue (three instructions).
+CM_HD u32 cm_sub(u32 a, u32 b, u32 p) {
+    const u32 d = a - b, e = d + p;
+    return e < d ? e : d;
+}
 
 CM_HD u32 cm_mulhi(u32 a, u32 b) {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -7,6 +7,7 @@
 // device memory; scratch space grows and is reused.
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -179,14 +180,28 @@ const Outer *outer_for(const Family *f, int k) {
     return o;
 }
 
-void *workspace(int slot, size_t bytes) {
-    static std::mutex m;
-    static std::map<int, std::pair<void *, size_t>> bufs;
-    std::lock_guard<std::mutex> lk(m);
-    auto &b = bufs[slot];
+namespace {
+std::mutex g_ws_mutex;
+std::map<std::pair<int, hipStream_t>, std::pair<void *, size_t>> g_ws;
+}  // namespace
+
+bool stream_capturing(hipStream_t s) {
+    if (!s) return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+void *workspace(int slot, size_t bytes, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    auto &b = g_ws[std::make_pair(slot, s)];
     if (b.second >= bytes && b.first) return b.first;
+    // a graph being captured cannot grow scratch (it would synchronise and
+    // free memory the graph refers to): the same calls must run once eagerly
+    // on this stream first
+    if (stream_capturing(s)) return nullptr;
     if (b.first) {
-        (void)hipDeviceSynchronize();
+        // work already queued on s may still use the old buffer
+        (void)hipStreamSynchronize(s);
         (void)hipFree(b.first);
         b.first = nullptr;
         b.second = 0;
@@ -197,6 +212,83 @@ void *workspace(int slot, size_t bytes) {
     b.first = d;
     b.second = sz;
     return d;
+}
+
+namespace {
+// Pinned staging for descriptor uploads, one per stream; reused once the
+// previous upload from it has finished.
+struct Staging {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+};
+std::map<hipStream_t, Staging> g_staging;   // guarded by g_ws_mutex
+// pinned buffers whose copies were recorded into a graph being captured on a
+// stream; they belong to that graph once the capture ends
+std::map<hipStream_t, std::vector<void *>> g_captured;
+}  // namespace
+
+hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    hipError_t e;
+    if (stream_capturing(s)) {
+        // the recorded copy must keep reading these bytes at every replay
+        void *h = nullptr;
+        e = hipHostMalloc(&h, bytes ? bytes : 1);
+        if (e != hipSuccess) return e;
+        memcpy(h, src, bytes);
+        g_captured[s].push_back(h);
+        return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s);
+    }
+    Staging &st = g_staging[s];
+    if (!st.done) {
+        e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    } else {
+        e = hipEventSynchronize(st.done);
+        if (e != hipSuccess) return e;
+    }
+    if (st.cap < bytes) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.cap = 0;
+        e = hipHostMalloc(&st.host, bytes);
+        if (e != hipSuccess) return e;
+        st.cap = bytes;
+    }
+    memcpy(st.host, src, bytes);
+    e = hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipEventRecord(st.done, s);
+}
+
+std::vector<void *> take_captured_buffers(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    std::vector<void *> v;
+    auto it = g_captured.find(s);
+    if (it != g_captured.end()) {
+        v.swap(it->second);
+        g_captured.erase(it);
+    }
+    return v;
+}
+
+void release_workspace(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    for (auto it = g_ws.begin(); it != g_ws.end();) {
+        if (it->first.second == s) {
+            if (it->second.first) (void)hipFree(it->second.first);
+            it = g_ws.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    auto st = g_staging.find(s);
+    if (st != g_staging.end()) {
+        if (st->second.done) (void)hipEventDestroy(st->second.done);
+        if (st->second.host) (void)hipHostFree(st->second.host);
+        g_staging.erase(st);
+    }
 }
 
 }  // namespace cm

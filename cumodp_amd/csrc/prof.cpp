@@ -65,6 +65,7 @@ bool prof_on() {
 ProfScope::ProfScope(int id, hipStream_t s) : id_(-1), s_(s) {
     std::lock_guard<std::mutex> lk(g_m);
     if (!enabled_locked() || id < 0 || id >= kProfCount) return;
+    if (stream_capturing(s)) return;   // graphs are timed as a whole by the caller
     hipEvent_t a;
     if (hipEventCreate(&a) != hipSuccess) return;
     if (hipEventRecord(a, s) != hipSuccess) { (void)hipEventDestroy(a); return; }

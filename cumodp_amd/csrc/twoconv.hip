@@ -77,11 +77,27 @@ static bool tc_choose(int bits, int64_t na, int64_t nb, TcPlan &pl) {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// Limb split: coefficient i of a (w words, two's complement) -> rows j < K of
-// out (row length Ly): |a_i| bits [jM, jM+M) mod p, negated mod p when
-// a_i < 0; rows K .. 2K-1 and positions i >= n are zero.
-__global__ void k_tc_split(const u64 *__restrict__ a, int64_t n, int w, int K, int K2, int M,
-                           u32 p, u32 *__restrict__ out, int64_t Ly) {
+// One product of a batch (word offsets into A, B, C; coefficient counts;
+// words per coefficient).
+struct TcDesc {
+    int64_t a_off, b_off, c_off, na, nb;
+    int32_t wa, wb, wc, pad;
+};
+
+// Limb split of one operand of a batch: coefficient i of item r (w words,
+// two's complement) -> out[(j R + r) Ly + i] for limb j < K: |a_i| bits
+// [jM, jM+M) mod p, negated mod p when a_i < 0; limbs K .. 2K-1 and
+// positions i >= n are zero.  grid.y: items (descriptor `one` when d is null).
+__global__ void k_tc_split(const u64 *__restrict__ src, const TcDesc *__restrict__ d,
+                           TcDesc one, int which, int K, int K2, int M, u32 p,
+                           u32 *__restrict__ out, int64_t Ly, int R) {
+    const int r = blockIdx.y;
+    const TcDesc dd = d ? d[r] : one;
+    const u64 *a = src + (which ? dd.b_off : dd.a_off);
+    const int64_t n = which ? dd.nb : dd.na;
+    const int w = which ? dd.wb : dd.wa;
+    u32 *o = out + (int64_t)r * Ly;
+    const int64_t js = (int64_t)R * Ly;           // stride between limb rows
     const u64 mask = (1ull << M) - 1;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Ly;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -102,7 +118,7 @@ __global__ void k_tc_split(const u64 *__restrict__ a, int64_t n, int w, int K, i
                 have += 64;
                 while (have >= M && j < K) {
                     const u32 limb = (u32)((u64)buf & mask);
-                    out[j * Ly + i] = neg ? (limb ? p - limb : 0u) : limb;
+                    o[j * js + i] = neg ? (limb ? p - limb : 0u) : limb;
                     buf >>= M;
                     have -= M;
                     ++j;
@@ -110,35 +126,42 @@ __global__ void k_tc_split(const u64 *__restrict__ a, int64_t n, int w, int K, i
             }
             for (; j < K; ++j) {                 // remaining bits, then zero limbs
                 const u32 limb = (u32)((u64)buf & mask);
-                out[j * Ly + i] = neg ? (limb ? p - limb : 0u) : limb;
+                o[j * js + i] = neg ? (limb ? p - limb : 0u) : limb;
                 buf >>= M;
             }
         }
-        for (; j < K2; ++j) out[j * Ly + i] = 0u;
+        for (; j < K2; ++j) o[j * js + i] = 0u;
     }
 }
 
-// CRT and recombination: output coefficient y = sum_j z_j 2^(M j), where z_j
-// (j < 2K-1) is the signed CRT value of the residues C_r[j][y].
+// CRT and recombination for a batch: output coefficient y of item r is
+// sum_j z_j 2^(M j), where z_j (j < 2K-1) is the signed CRT value of the
+// residues C_q[(j R + r) Ly + y].  grid.y: items.
 template <int NP>
 __global__ void k_tc_crt(const u32 *__restrict__ C0, const u32 *__restrict__ C1,
-                         const u32 *__restrict__ C2, int K2, int M, int64_t Ly, int64_t ny,
-                         u64 *__restrict__ out, int wc, u32 p0, u32 p1, u32 p2,
-                         u32 inv01, u32 inv01q, u32 inv012, u32 inv012q, u32 p0m2,
-                         u32 p0m2q) {
+                         const u32 *__restrict__ C2, const TcDesc *__restrict__ d, TcDesc one,
+                         int K2, int M, int64_t Ly, int R, u64 *__restrict__ out, u32 p0,
+                         u32 p1, u32 p2, u32 inv01, u32 inv01q, u32 inv012, u32 inv012q,
+                         u32 p0m2, u32 p0m2q) {
+    const int r = blockIdx.y;
+    const TcDesc dd = d ? d[r] : one;
+    const int64_t ny = dd.na + dd.nb - 1;
+    const int wc = dd.wc;
+    const int64_t js = (int64_t)R * Ly, base = (int64_t)r * Ly;
     for (int64_t y = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; y < ny;
          y += (int64_t)gridDim.x * blockDim.x) {
-        u64 *o = out + y * wc;
+        u64 *o = out + dd.c_off + y * wc;
         for (int q = 0; q < wc; ++q) o[q] = 0;
         for (int j = 0; j < K2 - 1; ++j) {
-            const u32 r0 = C0[j * Ly + y], r1 = C1[j * Ly + y];
+            const int64_t at = j * js + base + y;
+            const u32 r0 = C0[at], r1 = C1[at];
             // Garner: t1 = (r1 - r0) p0^-1 mod p1;  z = r0 + p0 t1 (+ p0 p1 t2)
             const u32 d1 = cm_sub(r1, r0 % p1, p1);
             const u32 t1 = cm_shoup(d1, inv01, inv01q, p1);
             unsigned __int128 z = (unsigned __int128)r0 + (unsigned __int128)p0 * t1;
             unsigned __int128 P = (unsigned __int128)p0 * p1;
             if (NP == 3) {
-                const u32 r2 = C2[j * Ly + y];
+                const u32 r2 = C2[at];
                 // t2 = (r2 - (r0 + p0 t1)) (p0 p1)^-1 mod p2
                 const u32 zm = (u32)(((u64)(r0 % p2) + cm_shoup(t1 % p2, p0m2, p0m2q, p2)) % p2);
                 const u32 d2 = cm_sub(r2, zm, p2);
@@ -229,6 +252,64 @@ int tc_bit_width(const u64 *a, int64_t n, int w) {
     return best;
 }
 
+// The products of one class (same plan and y size) over R items: per prime,
+// A and B are 2K x R x Ly (limb row j of item r at (j R + r) Ly), so the x
+// transforms are column transforms over R Ly columns and the y products are
+// 2K R rows of Ly words.  d: device descriptors, or null with `one`.
+static hipError_t tc_run_class(const TcPlan &pl, const u64 *A, const u64 *B, u64 *C,
+                               const TcDesc *d, const TcDesc &one, int R, int64_t ymax,
+                               u32 *ws, hipStream_t s) {
+    const int K2 = 1 << pl.lk2;
+    const int64_t Ly = (int64_t)1 << pl.ey;
+    const size_t per = (size_t)2 * K2 * R * Ly;   // A and B rows of one prime
+    const int bx = blocks_for(Ly);
+    u32 *Cr[3] = {nullptr, nullptr, nullptr};
+    for (int q = 0; q < pl.np; ++q) {
+        const u32 p = kTcPrimes[q];
+        const Family *fam = family_default(p);
+        if (!fam) return hipErrorOutOfMemory;
+        u32 *Aq = ws + per * q, *Bq = Aq + (size_t)K2 * R * Ly;
+        Cr[q] = Aq;
+        {
+            ProfScope ps(kProfTcSplit, s);
+            hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, A, d, one, 0, pl.K, K2,
+                               pl.M, p, Aq, Ly, R);
+            hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, B, d, one, 1, pl.K, K2,
+                               pl.M, p, Bq, Ly, R);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        // x: length-2K DFT down the columns (cyclic points even, negacyclic odd)
+        const u32 wx = cm_host_pow(fam->wN, (u64)1 << (fam->K - pl.lk2), p);
+        const Family *fx = family_for_root(p, wx, pl.lk2);
+        const Family *fxi = family_for_root(p, cm_host_inv(wx, p), pl.lk2);
+        if (!fx || !fxi) return hipErrorOutOfMemory;
+        e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, 1u, fx, s);
+        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Bq, (int64_t)R * Ly, 1u, fx, s);
+        // y: 2K R independent cyclic products of length Ly (= linear: ny <= Ly)
+        if (e == hipSuccess) e = launch_rows_mul(pl.ey, Aq, Bq, K2 * R, fam, s);
+        // inverse x transform with the 1/(2K) scaling
+        const u32 ki = cm_host_inv((u32)K2, p);
+        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, ki, fxi, s);
+        if (e != hipSuccess) return e;
+    }
+    const u32 p0 = kTcPrimes[0], p1 = kTcPrimes[1], p2 = kTcPrimes[2];
+    const u32 inv01 = cm_host_inv(p0 % p1, p1);
+    const u32 p0m2 = p0 % p2;
+    const u32 inv012 = cm_host_inv(cm_host_mul(p0 % p2, p1 % p2, p2), p2);
+    ProfScope ps(kProfTcCrt, s);
+    const dim3 g(blocks_for(ymax), R);
+    if (pl.np == 2)
+        hipLaunchKernelGGL((k_tc_crt<2>), g, dim3(256), 0, s, Cr[0], Cr[1], Cr[1], d, one, K2,
+                           pl.M, Ly, R, C, p0, p1, p2, inv01, cm_host_shoup_q(inv01, p1), inv012,
+                           cm_host_shoup_q(inv012, p2), p0m2, cm_host_shoup_q(p0m2, p2));
+    else
+        hipLaunchKernelGGL((k_tc_crt<3>), g, dim3(256), 0, s, Cr[0], Cr[1], Cr[2], d, one, K2,
+                           pl.M, Ly, R, C, p0, p1, p2, inv01, cm_host_shoup_q(inv01, p1), inv012,
+                           cm_host_shoup_q(inv012, p2), p0m2, cm_host_shoup_q(p0m2, p2));
+    return hipGetLastError();
+}
+
 // Runs the whole product on device buffers.  bits: largest coefficient bit
 // width of a and b (>= 1).
 hipError_t launch_two_convolution(const u64 *a, int64_t na, int wa, const u64 *b, int64_t nb,
@@ -239,59 +320,89 @@ hipError_t launch_two_convolution(const u64 *a, int64_t na, int wa, const u64 *b
         *err = 1;
         return hipSuccess;
     }
-    const int K2 = 1 << pl.lk2;
-    const int64_t Ly = (int64_t)1 << pl.ey;
-    const int64_t ny = na + nb - 1;
-    const size_t per = (size_t)2 * K2 * Ly;   // A and B rows of one prime
-    u32 *ws = (u32 *)workspace(10, sizeof(u32) * per * pl.np);
+    const size_t per = (size_t)2 * (1 << pl.lk2) * ((size_t)1 << pl.ey);
+    u32 *ws = (u32 *)workspace(10, sizeof(u32) * per * pl.np, s);
     if (!ws) return hipErrorOutOfMemory;
-    u32 *Cr[3] = {nullptr, nullptr, nullptr};
-    for (int r = 0; r < pl.np; ++r) {
-        const u32 p = kTcPrimes[r];
-        const Family *fam = family_default(p);
-        if (!fam) return hipErrorOutOfMemory;
-        u32 *A = ws + per * r, *B = A + (size_t)K2 * Ly;
-        Cr[r] = A;
-        {
-            ProfScope ps(kProfTcSplit, s);
-            hipLaunchKernelGGL(k_tc_split, dim3(blocks_for(Ly)), dim3(256), 0, s, a, na, wa, pl.K,
-                               K2, pl.M, p, A, Ly);
-            hipLaunchKernelGGL(k_tc_split, dim3(blocks_for(Ly)), dim3(256), 0, s, b, nb, wb, pl.K,
-                               K2, pl.M, p, B, Ly);
+    TcDesc one{0, 0, 0, na, nb, wa, wb, wc, 0};
+    return tc_run_class(pl, a, b, c, nullptr, one, 1, na + nb - 1, ws, s);
+}
+
+// Batch of products: items are grouped by plan (chosen for the worst case of
+// their y size, so one plan serves the class) and each class runs as one
+// sequence of batched launches, in chunks whose scratch stays under
+// kTcChunkBytes.
+static constexpr size_t kTcChunkBytes = (size_t)1 << 30;
+
+hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const u64 *A,
+                                       const u64 *B, u64 *C, hipStream_t s, int *err) {
+    *err = 0;
+    struct Cls {
+        TcPlan pl;
+        std::vector<int64_t> idx;
+    };
+    std::vector<Cls> classes;
+    for (int64_t i = 0; i < count; ++i) {
+        const TcItem &it = items[i];
+        int ey = 0;
+        while (((int64_t)1 << ey) < it.na + it.nb - 1) ++ey;
+        if (ey < 4) ey = 4;
+        // worst case of the class: n_min up to 2^(ey-1)
+        const int64_t nm = (int64_t)1 << (ey > 1 ? ey - 1 : 0);
+        TcPlan pl;
+        if (!tc_choose(it.bits, nm, nm, pl) || pl.lk2 > kMaxLdsLog) {
+            *err = 1;
+            return hipSuccess;
         }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        // x: length-2K DFT over the rows (cyclic points even, negacyclic odd)
-        const u32 wx = cm_host_pow(fam->wN, (u64)1 << (fam->K - pl.lk2), p);
-        const Family *fx = family_for_root(p, wx, pl.lk2);
-        const Family *fxi = family_for_root(p, cm_host_inv(wx, p), pl.lk2);
-        if (!fx || !fxi) return hipErrorOutOfMemory;
-        e = launch_col_ntt(pl.lk2, A, Ly, 1u, fx, s);
-        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, B, Ly, 1u, fx, s);
-        // y: 2K independent cyclic products of length Ly (= linear: ny <= Ly)
-        if (e == hipSuccess) e = launch_rows_mul(pl.ey, A, B, K2, fam, s);
-        // inverse x transform with the 1/(2K) scaling
-        const u32 ki = cm_host_inv((u32)K2, p);
-        if (e == hipSuccess)
-            e = launch_col_ntt(pl.lk2, A, Ly, ki, fxi, s);
-        if (e != hipSuccess) return e;
+        pl.ey = ey;
+        if (pl.ey > kMaxLog) {
+            *err = 1;
+            return hipSuccess;
+        }
+        Cls *c = nullptr;
+        for (auto &x : classes)
+            if (x.pl.np == pl.np && x.pl.K == pl.K && x.pl.M == pl.M && x.pl.ey == pl.ey) c = &x;
+        if (!c) {
+            classes.push_back(Cls{pl, {}});
+            c = &classes.back();
+        }
+        c->idx.push_back(i);
     }
-    const u32 p0 = kTcPrimes[0], p1 = kTcPrimes[1], p2 = kTcPrimes[2];
-    const u32 inv01 = cm_host_inv(p0 % p1, p1);
-    const u32 p0m2 = p0 % p2;
-    const u32 inv012 = cm_host_inv(cm_host_mul(p0 % p2, p1 % p2, p2), p2);
-    ProfScope ps(kProfTcCrt, s);
-    if (pl.np == 2)
-        hipLaunchKernelGGL((k_tc_crt<2>), dim3(blocks_for(ny)), dim3(256), 0, s, Cr[0], Cr[1],
-                           Cr[1], K2, pl.M, Ly, ny, c, wc, p0, p1, p2, inv01,
-                           cm_host_shoup_q(inv01, p1), inv012, cm_host_shoup_q(inv012, p2), p0m2,
-                           cm_host_shoup_q(p0m2, p2));
-    else
-        hipLaunchKernelGGL((k_tc_crt<3>), dim3(blocks_for(ny)), dim3(256), 0, s, Cr[0], Cr[1],
-                           Cr[2], K2, pl.M, Ly, ny, c, wc, p0, p1, p2, inv01,
-                           cm_host_shoup_q(inv01, p1), inv012, cm_host_shoup_q(inv012, p2), p0m2,
-                           cm_host_shoup_q(p0m2, p2));
-    return hipGetLastError();
+    // descriptors of every class, in class order, uploaded once
+    std::vector<TcDesc> hd((size_t)count);
+    size_t at = 0;
+    for (auto &c : classes)
+        for (int64_t i : c.idx) {
+            const TcItem &it = items[i];
+            hd[at++] = TcDesc{it.a_off, it.b_off, it.c_off, it.na, it.nb, it.wa, it.wb, it.wc, 0};
+        }
+    TcDesc *dd = (TcDesc *)workspace(14, sizeof(TcDesc) * hd.size(), s);
+    if (!dd) return hipErrorOutOfMemory;
+    hipError_t e = stage_upload(hd.data(), sizeof(TcDesc) * hd.size(), dd, s);
+    if (e != hipSuccess) return e;
+    size_t first = 0;
+    for (auto &c : classes) {
+        const int K2 = 1 << c.pl.lk2;
+        const int64_t Ly = (int64_t)1 << c.pl.ey;
+        const size_t item_bytes = sizeof(u32) * (size_t)2 * K2 * Ly * c.pl.np;
+        int64_t chunk = (int64_t)(kTcChunkBytes / item_bytes);
+        if (chunk < 1) chunk = 1;
+        const int64_t n = (int64_t)c.idx.size();
+        for (int64_t k0 = 0; k0 < n; k0 += chunk) {
+            const int R = (int)std::min<int64_t>(chunk, n - k0);
+            u32 *ws = (u32 *)workspace(10, item_bytes * (size_t)R, s);
+            if (!ws) return hipErrorOutOfMemory;
+            int64_t ymax = 1;
+            for (int r = 0; r < R; ++r) {
+                const TcItem &it = items[c.idx[(size_t)(k0 + r)]];
+                ymax = std::max<int64_t>(ymax, it.na + it.nb - 1);
+            }
+            TcDesc none{};
+            e = tc_run_class(c.pl, A, B, C, dd + first + k0, none, R, ymax, ws, s);
+            if (e != hipSuccess) return e;
+        }
+        first += (size_t)n;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_scale_x(u32 *A, int64_t len, int K, u32 theta, u32 p, hipStream_t s) {
@@ -302,7 +413,7 @@ hipError_t launch_scale_x(u32 *A, int64_t len, int K, u32 theta, u32 p, hipStrea
         tp[(size_t)j] = make_uint2((u32)x, cm_host_shoup_q((u32)x, p));
         x = x * (theta % p) % p;
     }
-    uint2 *d = (uint2 *)workspace(11, sizeof(uint2) * (size_t)K);
+    uint2 *d = (uint2 *)workspace(11, sizeof(uint2) * (size_t)K, s);
     if (!d) return hipErrorOutOfMemory;
     hipError_t e = hipMemcpy(d, tp.data(), sizeof(uint2) * (size_t)K, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;

@@ -25,6 +25,9 @@ CUMODP_API cumodp_err cumodp_synchronize(void);
 CUMODP_API void *cumodp_stream_create(void);
 CUMODP_API cumodp_err cumodp_stream_destroy(void *stream);
 CUMODP_API cumodp_err cumodp_stream_synchronize(void *stream);
+/* 1 when all work queued on the stream has finished, 0 while some is
+ * pending, -1 on error (non-blocking) */
+CUMODP_API int cumodp_stream_query(void *stream);
 /* error string of the last failing device call in this thread ("" if none) */
 CUMODP_API const char *cumodp_last_error(void);
 
@@ -76,6 +79,31 @@ CUMODP_API cumodp_err cumodp_two_convolution_poly_mul_dev(const uint64_t *a, int
                                                           const uint64_t *b, int64_t nb, int wb,
                                                           uint64_t *c, int wc, int max_bits,
                                                           void *stream);
+/* Many exact integer products in one call (the serving path).  Item i reads
+ * na coefficients of wa words at A + a_off and nb of wb words at B + b_off
+ * (two's complement, little-endian words) and writes the na+nb-1 product
+ * coefficients of wc words at C + c_off (offsets in 64-bit words).  bits:
+ * the largest coefficient bit width of the item's operands (>= 1); wc*64 must
+ * cover 2*bits + ceil(log2(min(na, nb))) + 1.  Items are grouped by
+ * transform plan and each group runs as one batch of launches.  Output ranges
+ * must not overlap each other or the inputs.  Stream ordered.  No counterpart
+ * in the reference (its two_convolution_poly_mul takes one product). */
+typedef struct {
+    int64_t a_off, b_off, c_off;
+    int64_t na, nb;
+    int32_t wa, wb, wc, bits;
+} cumodp_zmul_item;
+CUMODP_API cumodp_err cumodp_two_convolution_list_dev(const cumodp_zmul_item *items, int64_t count,
+                                                      const uint64_t *A, const uint64_t *B,
+                                                      uint64_t *C, void *stream);
+/* pinned host memory for staging (hipHostMalloc); NULL on failure */
+CUMODP_API void *cumodp_host_alloc(size_t bytes);
+CUMODP_API cumodp_err cumodp_host_free(void *ptr);
+/* asynchronous copies on a stream (pinned host memory for overlap) */
+CUMODP_API cumodp_err cumodp_memcpy_h2d_async(void *dst, const void *src, size_t bytes,
+                                              void *stream);
+CUMODP_API cumodp_err cumodp_memcpy_d2h_async(void *dst, const void *src, size_t bytes,
+                                              void *stream);
 /* find_largest_bit_width_of_coefficients (src/two_convolution_poly_mul.cu:58-70):
  * the largest mpz_sizeinbase(|x|, 2) over a and b (1 for zero) */
 CUMODP_API int cumodp_find_largest_bit_width(const uint64_t *a, int64_t na, int wa,
@@ -106,11 +134,35 @@ CUMODP_API cumodp_err cumodp_subproduct_tree_dev(const sfixn *points, int k, sfi
                                                  void *stream);
 CUMODP_API cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p, sfixn *tree);
 
+/* ---- graphs of repeated shapes --------------------------------------------
+ * Records the stream-ordered cumodp_*_dev calls made on `stream` (a stream
+ * from cumodp_stream_create, not the NULL stream) between begin and end into
+ * a HIP graph that replays them with one launch.  Run the same calls once
+ * eagerly on that stream before capturing (scratch buffers and twiddle
+ * tables cannot be allocated inside a capture: such a call then fails).  A
+ * replay reads the device buffers named at capture time and the descriptors
+ * as they were then; the graph owns the pinned copies of those descriptors.
+ * No memset nodes are recorded (the library clears memory with kernels). */
+CUMODP_API cumodp_err cumodp_graph_begin(void *stream);
+CUMODP_API void *cumodp_graph_end(void *stream);
+CUMODP_API cumodp_err cumodp_graph_launch(void *graph, void *stream);
+CUMODP_API cumodp_err cumodp_graph_destroy(void *graph);
 /* ---- tuning: four-step split ----------------------------------------------
  * Transforms of 2^k points above 2^13 run as 2^(k-k2) columns x 2^k2 rows;
  * this chooses k2 (12, 13 or 14) for one k.  The tuner (cumodp_amd/autotune.py)
  * measures the candidates and records its choice. */
 CUMODP_API cumodp_err cumodp_tune_set_split(int k, int k2);
+/* the current split k2 of 2^k-point transforms (0 when k needs none) */
+CUMODP_API int cumodp_tune_get_split(int k);
+/* words per workgroup tile of the list multiply's mixed-size kernel: 2048
+ * (128 threads), 4096 (256, the default) or 8192 (512); products of up to that
+ * many points share tiles, larger ones run one per workgroup or four-step */
+CUMODP_API cumodp_err cumodp_tune_set_list_tile(int words);
+CUMODP_API int cumodp_tune_get_list_tile(void);
+/* words per workgroup tile of the four-step multiply's column passes: 16384
+ * (the default; 1024 threads) or 8192 (512 threads, more workgroups) */
+CUMODP_API cumodp_err cumodp_tune_set_col_tile(int words);
+CUMODP_API int cumodp_tune_get_col_tile(void);
 
 /* ---- optional per-kernel timing ------------------------------------------
  * Off unless enabled here or with CUMODP_PROFILE=1.  When on, each hot kernel
