@@ -1,0 +1,56 @@
+"""Host-side logic of the serving loop (no GPU): output widths, admission by
+footprint, latency summary."""
+import types
+
+import numpy as np
+
+from cumodp_amd import serving
+
+
+def test_out_words_cover_the_product_bound():
+    # 2*bits + ceil(log2(min(na, nb))) + 1 bits must fit
+    assert serving.out_words(16, 65, 70) == 1        # 32 + 7 + 1 = 40
+    assert serving.out_words(32, 1000, 10) == 2      # 64 + 4 + 1 = 69
+    assert serving.out_words(48, 65537, 65537) == 2  # 96 + 17 + 1 = 114
+    assert serving.out_words(64, 65537, 65537) == 3  # 128 + 17 + 1 = 146
+    assert serving.out_words(64, 1, 1) == 3          # 128 + 0 + 1 = 129
+    assert serving.out_words(31, 1, 1) == 1          # 62 + 1 = 63
+
+
+def _fake_requests(na, nb, bits):
+    na, nb, bits = (np.asarray(x, np.int64) for x in (na, nb, bits))
+    Q = types.SimpleNamespace(na=na, nb=nb, bits=bits, nc=na + nb - 1)
+    Q.wc = np.array([serving.out_words(int(b), int(x), int(y)) for b, x, y in zip(bits, na, nb)])
+    return Q
+
+
+def _server(cap):
+    s = object.__new__(serving.ZmulServer)
+    s.slot_bytes = cap
+    s._plans = {}
+    # scratch estimate without the library: 2 x 8 x Ly words per prime, 2 primes
+    s._scratch_bytes = lambda bits, ny: 4 * 2 * 8 * (1 << max(4, (int(ny) - 1).bit_length())) * 2
+    return s
+
+
+def test_admission_respects_the_footprint_cap():
+    Q = _fake_requests([100] * 10, [100] * 10, [64] * 10)
+    one = 8 * (200 + 199 * 3) + 4 * 2 * 8 * 256 * 2
+    s = _server(3 * one)
+    assert s._fit(Q, 0, 10) == 3
+    assert s._fit(Q, 3, 10) == 6
+    assert s._fit(Q, 9, 10) == 10
+    # a request larger than the cap is admitted alone
+    s = _server(one // 2)
+    assert s._fit(Q, 0, 10) == 1
+    # only arrived requests are considered
+    s = _server(100 * one)
+    assert s._fit(Q, 2, 5) == 5
+
+
+def test_latency_summary():
+    arr = np.array([0.0, 0.001, 0.002, 0.003])
+    done = arr + np.array([0.002, 0.004, 0.001, 0.003])
+    s = serving.latency_summary(done, arr)
+    assert abs(s["p50_ms"] - 2.5) < 1e-9
+    assert abs(s["max_ms"] - 4.0) < 1e-9
