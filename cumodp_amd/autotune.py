@@ -28,16 +28,24 @@ from . import CumodpError, DeviceArray, _check, lib, list_fftmul_dev, make_items
 from .workload import P2, SplitMix64, uniform_mod, workload4
 
 
-def _median_time(fn, reps):
+def _median_time(fn, reps, min_s=2e-3):
+    """Median over reps of the time per call of fn, each rep running enough
+    back-to-back calls (stream ordered, one synchronisation) to last min_s,
+    so that launch and synchronisation latency do not decide the choice."""
     fn()
     synchronize()
+    t0 = time.perf_counter()
+    fn()
+    synchronize()
+    inner = max(1, int(min_s / max(time.perf_counter() - t0, 1e-6)) + 1)
     ts = []
     for _ in range(reps):
         synchronize()
         t0 = time.perf_counter()
-        fn()
+        for _ in range(inner):
+            fn()
         synchronize()
-        ts.append(time.perf_counter() - t0)
+        ts.append((time.perf_counter() - t0) / inner)
     return float(np.median(ts))
 
 

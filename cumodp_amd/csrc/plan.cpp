@@ -115,9 +115,10 @@ void four_step_split(int k, int &k1, int &k2) {
         const char *e = getenv(name);
         if (e) c = atoi(e);
     }
-    // measured default (2^24 on MI355X: 13 beats 12 and 14): wider column
-    // tiles once the columns are long enough
-    if (c < 12 || c > 14 || k - c < 1 || k - c > kMaxLdsLog) c = k >= 21 ? 13 : 12;
+    // measured defaults on MI355X (cumodp_amd.autotune, back-to-back 2^k
+    // multiplies): 2^12-word rows up to 2^20, 2^13 for 2^21..2^24, 2^14 above
+    if (c < 12 || c > 14 || k - c < 1 || k - c > kMaxLdsLog)
+        c = k >= 25 ? 14 : (k >= 21 ? 13 : 12);
     k2 = c;
     k1 = k - k2;
 }

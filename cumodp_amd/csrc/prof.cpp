@@ -30,6 +30,21 @@ struct Rec {
 
 std::mutex g_m;
 std::vector<Rec> g_open, g_done;
+std::vector<hipEvent_t> g_pool;   // recorded-and-read events, reused
+
+// Timing events without the system-scope release fence (the bracketed
+// kernels' results stay in device memory), taken from a pool so that no
+// event is created inside a timed region once the pool is warm.
+hipEvent_t take_event_locked() {
+    if (!g_pool.empty()) {
+        hipEvent_t e = g_pool.back();
+        g_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
+    return e;
+}
 double g_total[kProfCount];
 int64_t g_count[kProfCount];
 int g_enabled = -1;
@@ -49,8 +64,8 @@ void drain_locked() {
             g_total[r.id] += ms;
             g_count[r.id] += 1;
         }
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
+        g_pool.push_back(r.a);
+        g_pool.push_back(r.b);
     }
     g_done.clear();
 }
@@ -66,9 +81,9 @@ ProfScope::ProfScope(int id, hipStream_t s) : id_(-1), s_(s) {
     std::lock_guard<std::mutex> lk(g_m);
     if (!enabled_locked() || id < 0 || id >= kProfCount) return;
     if (stream_capturing(s)) return;   // graphs are timed as a whole by the caller
-    hipEvent_t a;
-    if (hipEventCreate(&a) != hipSuccess) return;
-    if (hipEventRecord(a, s) != hipSuccess) { (void)hipEventDestroy(a); return; }
+    hipEvent_t a = take_event_locked();
+    if (!a) return;
+    if (hipEventRecord(a, s) != hipSuccess) { g_pool.push_back(a); return; }
     id_ = id;
     ev_ = (void *)a;
 }
@@ -76,9 +91,9 @@ ProfScope::ProfScope(int id, hipStream_t s) : id_(-1), s_(s) {
 ProfScope::~ProfScope() {
     if (id_ < 0) return;
     std::lock_guard<std::mutex> lk(g_m);
-    hipEvent_t b;
-    if (hipEventCreate(&b) != hipSuccess) return;
-    if (hipEventRecord(b, s_) != hipSuccess) { (void)hipEventDestroy(b); return; }
+    hipEvent_t b = take_event_locked();
+    if (!b) return;
+    if (hipEventRecord(b, s_) != hipSuccess) { g_pool.push_back(b); return; }
     g_done.push_back(Rec{id_, (hipEvent_t)ev_, b});
     if (g_done.size() > 4096) drain_locked();
 }

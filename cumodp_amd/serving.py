@@ -38,6 +38,12 @@ def out_words(bits, na, nb):
     return (2 * bits + lg + 1 + 63) // 64
 
 
+_ITEM_DTYPE = np.dtype([("a_off", np.int64), ("b_off", np.int64), ("c_off", np.int64),
+                        ("na", np.int64), ("nb", np.int64), ("wa", np.int32), ("wb", np.int32),
+                        ("wc", np.int32), ("bits", np.int32)])
+assert _ITEM_DTYPE.itemsize == C.sizeof(ZmulItem)
+
+
 class PinnedWords:
     """A pinned host buffer of uint64 words (hipHostMalloc) with a numpy view."""
 
@@ -79,6 +85,15 @@ class Requests:
         self.in_off[1:] = np.cumsum(self.na + self.nb)
         self.out_off = np.zeros(n + 1, np.int64)
         self.out_off[1:] = np.cumsum(self.nc * self.wc)
+        # per-request item fields (cumodp_zmul_item layout); offsets are made
+        # relative to a batch's buffers when it launches
+        self.items = np.zeros(n, _ITEM_DTYPE)
+        self.items["a_off"] = self.in_off[:-1]
+        self.items["b_off"] = self.in_off[:-1] + self.na
+        self.items["c_off"] = self.out_off[:-1]
+        self.items["na"], self.items["nb"] = self.na, self.nb
+        self.items["wa"] = self.items["wb"] = 1
+        self.items["wc"], self.items["bits"] = self.wc, self.bits
         self.IN = PinnedWords(int(self.in_off[-1]))
         self.OUT = PinnedWords(int(self.out_off[-1]))
         for r in range(n):
@@ -193,19 +208,13 @@ class ZmulServer:
         _check("cumodp_memcpy_h2d_async", L.cumodp_memcpy_h2d_async(
             C.c_void_p(slot.din), C.c_void_p(Q.IN.ptr + 8 * in0), 8 * (in1 - in0), st))
         n = i1 - i0
-        items = (ZmulItem * n)()
-        for k in range(n):
-            r = i0 + k
-            it = items[k]
-            it.a_off = int(Q.in_off[r]) - in0
-            it.b_off = it.a_off + int(Q.na[r])
-            it.c_off = int(Q.out_off[r]) - out0
-            it.na, it.nb = int(Q.na[r]), int(Q.nb[r])
-            it.wa = it.wb = 1
-            it.wc = int(Q.wc[r])
-            it.bits = int(Q.bits[r])
+        items = Q.items[i0:i1].copy()
+        items["a_off"] -= in0
+        items["b_off"] -= in0
+        items["c_off"] -= out0
         _check("cumodp_two_convolution_list_dev", L.cumodp_two_convolution_list_dev(
-            items, n, C.c_void_p(slot.din), C.c_void_p(slot.din), C.c_void_p(slot.dout), st))
+            items.ctypes.data_as(C.POINTER(ZmulItem)), n, C.c_void_p(slot.din),
+            C.c_void_p(slot.din), C.c_void_p(slot.dout), st))
         _check("cumodp_memcpy_d2h_async", L.cumodp_memcpy_d2h_async(
             C.c_void_p(Q.OUT.ptr + 8 * out0), C.c_void_p(slot.dout), 8 * (out1 - out0), st))
         slot.range = (i0, i1)
