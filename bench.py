@@ -180,6 +180,41 @@ def bench_tree(cm, cpu, reps=5):
     return out
 
 
+def bench_eval(cm, cpu, reps=5):
+    """Fast multipoint evaluation of a degree-(2^16-1) polynomial at the 2^16
+    tree points mod 469762049 (tree, subinverse tree, remainder tree and
+    Horner at the leaves).  CPU leg: the oracle's Horner on a sample of the
+    points, checked against the GPU values."""
+    w = cm.workload4_tree()
+    p, pts = w["p"], w["points"]
+    k = 16
+    F = cm.uniform_mod(cm.SplitMix64(616), 1 << k, p)
+    L = cm.lib()
+    dF, dP, dV = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(pts), cm.DeviceArray(1 << k)
+    run = lambda: cm._check("cumodp_fast_evaluation_dev", L.cumodp_fast_evaluation_dev(
+        dF.addr(), dP.addr(), k, p, dV.addr(), None))
+    run()
+    t = timed(run, reps, cm.synchronize)
+    vals = dV.download()
+    for d in (dF, dP, dV):
+        d.free()
+    out = {"workload": "evaluate a degree-(2^16-1) polynomial at 2^16 points mod 469762049",
+           "ms": t * 1e3, "points_per_s": (1 << k) / t}
+    if cpu:
+        import oracle
+        oracle.lib()
+        idx = np.arange(0, 1 << k, 64)
+        t0 = time.perf_counter()
+        want = oracle.poly_eval(F, pts[idx], p)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": len(idx) / tc, "unit": "points/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{len(idx)} of the points by the oracle's Horner "
+                                         "rule (2^16 terms each)",
+                               "check_bit_exact": bool(np.array_equal(vals[idx], want))}
+    return out
+
+
 def _prod_mod(vals, p):
     acc = 1
     for v in vals:
@@ -474,6 +509,7 @@ def main():
         sec = {}
         cpu = not args.no_cpu_baseline
         for key, fn in (("list_multiply", bench_list_multiply), ("subproduct_tree", bench_tree),
+                        ("fast_evaluation", bench_eval),
                         ("two_convolution", bench_two_convolution)):
             log(f"bench: {key}")
             sec[key] = fn(cm, cpu)

@@ -169,6 +169,10 @@ def lib():
         L.cumodp_subproduct_tree_dev.restype = err
         L.cumodp_subproduct_tree.argtypes = [vp, s, s, vp]
         L.cumodp_subproduct_tree.restype = err
+        L.cumodp_fast_evaluation.argtypes = [vp, vp, s, s, vp]
+        L.cumodp_fast_evaluation.restype = err
+        L.cumodp_fast_evaluation_dev.argtypes = [vp, vp, s, s, vp, vp]
+        L.cumodp_fast_evaluation_dev.restype = err
         L.cumodp_two_convolution_list_dev.argtypes = [C.POINTER(ZmulItem), i64, vp, vp, vp, vp]
         L.cumodp_two_convolution_list_dev.restype = err
         L.cumodp_host_alloc.argtypes = [C.c_size_t]
@@ -400,6 +404,19 @@ def subproduct_tree(points, p):
     for i in range(k + 1):
         off = L.cumodp_subproduct_tree_offset(k, i)
         out.append(tree[off:off + (1 << k)  + (1 << (k - i))].reshape(1 << (k - i), (1 << i) + 1))
+    return out
+
+
+def fast_evaluation(F, points, p):
+    """F(points[j]) mod p for len(points) = 2^k points, F of degree < 2^k."""
+    pts = _i32(points)
+    k = len(pts).bit_length() - 1
+    assert len(pts) == 1 << k
+    Fw = np.zeros(1 << k, np.int32)
+    Fw[:len(F)] = _i32(F)
+    out = np.zeros(1 << k, np.int32)
+    _check("cumodp_fast_evaluation", lib().cumodp_fast_evaluation(_ptr(Fw), _ptr(pts), k, p,
+                                                                  _ptr(out)))
     return out
 
 

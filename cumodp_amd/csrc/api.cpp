@@ -547,6 +547,45 @@ cumodp_err cumodp_subproduct_tree_dev(const sfixn *points, int k, sfixn p, sfixn
     return CUMODP_SUCCESS;
 }
 
+static cumodp_err eval_impl(const u32 *F, const u32 *pts, int k, u32 p, u32 *values,
+                            hipStream_t s) {
+    const Family *fam = family_default(p);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    u32 *scratch = (u32 *)workspace(15, sizeof(u32) * (size_t)eval_scratch_words(k), s);
+    if (!scratch) return fail(CUMODP_OUT_OF_MEMORY, "evaluation scratch");
+    hipError_t he = launch_fast_evaluation(F, pts, k, p, values, scratch, fam, s);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "fast evaluation", he);
+    return CUMODP_SUCCESS;
+}
+
+cumodp_err cumodp_fast_evaluation_dev(const sfixn *F, const sfixn *points, int k, sfixn p,
+                                      sfixn *values, void *stream) {
+    cumodp_err ce = tree_check(k, p);
+    if (ce != CUMODP_SUCCESS) return ce;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return eval_impl((const u32 *)F, (const u32 *)points, k, (u32)p, (u32 *)values,
+                     as_stream(stream));
+}
+
+cumodp_err cumodp_fast_evaluation(const sfixn *F, const sfixn *points, int k, sfixn p,
+                                  sfixn *values) {
+    cumodp_err ce = tree_check(k, p);
+    if (ce != CUMODP_SUCCESS) return ce;
+    const int64_t n = (int64_t)1 << k;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    u32 *d = (u32 *)workspace(16, sizeof(u32) * (size_t)(3 * n), 0);
+    if (!d) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    hipError_t he = hipMemcpy(d, F, sizeof(u32) * (size_t)n, hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+        he = hipMemcpy(d + n, points, sizeof(u32) * (size_t)n, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    ce = eval_impl(d, d + n, k, (u32)p, d + 2 * n, 0);
+    if (ce != CUMODP_SUCCESS) return ce;
+    he = hipMemcpy(values, d + 2 * n, sizeof(u32) * (size_t)n, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    return CUMODP_SUCCESS;
+}
+
 cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p, sfixn *tree) {
     cumodp_err ce = tree_check(k, p);
     if (ce != CUMODP_SUCCESS) return ce;

@@ -1,0 +1,221 @@
+// eval.hip -- fast multipoint evaluation of a polynomial at 2^k points.
+//
+// Reference: subInvTree and fastEvaluation (src/fastPolyEvaluation.cu:666-1400,
+// driver :1371-1400).  The reference evaluates F of degree < 2^k at 2^k points
+// by going down the subproduct tree, replacing each remainder by its
+// remainders modulo the two children.  It uses Newton-iteration inverses of
+// the reversed node polynomials (the "subinverse tree") for the divisions
+// above level plainMulLimit and plain division below.  Here:
+//   * the subproduct tree is built by launch_subproduct_tree (tree.hip);
+//   * subinverse tree: I_i(j) = rev(M_i(j))^-1 mod x^(2^i).  At level L0 it
+//     comes from a direct power-series inversion (one thread per node).
+//     Above that, I_(i+1)(j) comes from the two children: their product is
+//     the inverse mod x^(2^i), and one Newton step g <- g (2 - f g) doubles
+//     the precision to 2^(i+1).  That is three batched products per level;
+//   * remainder tree: R_k = F, and R_i(j) = R_(i+1)(j/2) mod M_i(j) by
+//     q = rev(rev(A) I_i(j) mod x^h) and R = (A - q M) mod x^h with h = 2^i.
+//     That is two batched products per level;
+//   * at level L0 each remainder (2^L0 coefficients) is evaluated at its
+//     node's 2^L0 points by Horner's rule.
+// Every batched product of a level is one launch_list_mul call (mixed-size
+// LDS kernel for small nodes, four-step for large ones).
+#include <vector>
+
+#include "internal.h"
+
+namespace cm {
+
+constexpr int kEvalBase = 4;   // L0: levels below use plain arithmetic
+
+static int eblocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
+// I_L0(j) = rev(M)^-1 mod x^h for the nodes of level L0 (h = 2^L0): with
+// f = rev(M) (f_0 = 1, f_s = M_(h-s)), g_0 = 1 and g_t = -sum_(s=1..t) f_s g_(t-s).
+__global__ void k_inv_base(const u32 *__restrict__ M, int h, int64_t nodes, u32 p,
+                           u32 *__restrict__ inv) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nodes;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const u32 *m = M + j * (h + 1);
+        u32 *g = inv + j * h;
+        g[0] = 1u;
+        for (int t = 1; t < h; ++t) {
+            u64 acc = 0;
+            for (int s = 1; s <= t; ++s) acc = (acc + (u64)m[h - s] * g[t - s]) % p;
+            g[t] = acc ? p - (u32)acc : 0u;
+        }
+    }
+}
+
+// f(j) = rev(M_(i+1)(j)) mod x^(2h): f_t = M_(2h - t), t < 2h (node length 2h + 1)
+__global__ void k_rev_node(const u32 *__restrict__ M, int h2, int64_t nodes,
+                           u32 *__restrict__ f) {
+    const int64_t total = nodes * h2;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = g / h2;
+        const int t = (int)(g % h2);
+        f[g] = M[j * (h2 + 1) + (h2 - t)];
+    }
+}
+
+// u = 2 - e (mod p), in place
+__global__ void k_two_minus(u32 *__restrict__ e, int h2, int64_t total, u32 p) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const u32 v = e[g];
+        if (g % h2 == 0) e[g] = cm_sub(2u % p, v, p);
+        else e[g] = v ? p - v : 0u;
+    }
+}
+
+// reversed top half of each child's dividend: out(j)_t = A(j/2)_(2h-1-t), t < h
+__global__ void k_div_rev_top(const u32 *__restrict__ A, int h, int64_t nodes,
+                              u32 *__restrict__ out) {
+    const int64_t total = nodes * h;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = g / h;
+        const int t = (int)(g % h);
+        out[g] = A[(j >> 1) * (2 * h) + (2 * h - 1 - t)];
+    }
+}
+
+// q(j)_t = qrev(j)_(h-1-t)
+__global__ void k_reverse(const u32 *__restrict__ in, int h, int64_t nodes,
+                          u32 *__restrict__ out) {
+    const int64_t total = nodes * h;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = g / h;
+        const int t = (int)(g % h);
+        out[g] = in[j * h + (h - 1 - t)];
+    }
+}
+
+// R(j)_t = A(j/2)_t - qm(j)_t, t < h
+__global__ void k_div_finish(const u32 *__restrict__ A, const u32 *__restrict__ qm, int h,
+                             int64_t nodes, u32 p, u32 *__restrict__ R) {
+    const int64_t total = nodes * h;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = g / h;
+        const int t = (int)(g % h);
+        R[g] = cm_sub(A[(j >> 1) * (2 * h) + t], qm[g], p);
+    }
+}
+
+// values_x = R_L0(x / h)(point x) by Horner's rule
+__global__ void k_horner(const u32 *__restrict__ R, int h, const u32 *__restrict__ pts,
+                         int64_t n, u32 p, u32 *__restrict__ val) {
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const u32 *r = R + (x / h) * h;
+        const u64 z = pts[x] % p;
+        u64 acc = 0;
+        for (int t = h - 1; t >= 0; --t) acc = (acc * z + r[t]) % p;
+        val[x] = (u32)acc;
+    }
+}
+
+int64_t eval_scratch_words(int k) {
+    const int64_t n = (int64_t)1 << k;
+    const int L0 = k < kEvalBase ? k : kEvalBase;
+    return tree_level_offset(k, k + 1) + (int64_t)(k - L0 + 1) * n + 5 * n;
+}
+
+hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u32 *values,
+                                  u32 *scratch, const Family *fam, hipStream_t s) {
+    const int64_t n = (int64_t)1 << k;
+    const int L0 = k < kEvalBase ? k : kEvalBase;
+    u32 *tree = scratch;
+    u32 *inv = tree + tree_level_offset(k, k + 1);           // levels L0 .. k, n words each
+    u32 *R0 = inv + (int64_t)(k - L0 + 1) * n, *R1 = R0 + n;
+    u32 *T1 = R1 + n, *T2 = T1 + n, *T3 = T2 + n;
+    auto inv_at = [&](int i) { return inv + (int64_t)(i - L0) * n; };
+    auto node = [&](int i) { return tree + tree_level_offset(k, i); };
+
+    hipError_t e = launch_subproduct_tree(pts, k, p, tree, fam, s);
+    if (e != hipSuccess) return e;
+    // ---- subinverse tree, levels L0 .. k-1
+    {
+        const int h = 1 << L0;
+        hipLaunchKernelGGL(k_inv_base, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0), h,
+                           n >> L0, p, inv_at(L0));
+        e = hipGetLastError();
+    }
+    std::vector<MulDesc> d;
+    for (int i = L0; i + 1 <= k - 1 && e == hipSuccess; ++i) {
+        const int h = 1 << i;
+        const int64_t N = n >> (i + 1);                      // nodes of level i + 1
+        // g0(j) = I_i(2j) I_i(2j+1) mod x^h
+        d.assign((size_t)N, MulDesc{});
+        for (int64_t j = 0; j < N; ++j) {
+            MulDesc &m = d[(size_t)j];
+            m.f = (2 * j) * h; m.g = (2 * j + 1) * h; m.h = j * h;
+            m.lf = m.lg = h; m.lh = h; m.logn = i + 1;
+        }
+        e = launch_list_mul(d.data(), N, inv_at(i), inv_at(i), T1, fam, s);
+        if (e != hipSuccess) break;
+        // f(j) = rev(M_(i+1)(j)) mod x^(2h)
+        hipLaunchKernelGGL(k_rev_node, dim3(eblocks(N * 2 * h)), dim3(256), 0, s, node(i + 1),
+                           2 * h, N, T2);
+        // e(j) = f(j) g0(j) mod x^(2h)
+        for (int64_t j = 0; j < N; ++j) {
+            MulDesc &m = d[(size_t)j];
+            m.f = j * 2 * h; m.g = j * h; m.h = j * 2 * h;
+            m.lf = 2 * h; m.lg = h; m.lh = 2 * h; m.logn = i + 2;
+        }
+        e = launch_list_mul(d.data(), N, T2, T1, T3, fam, s);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(k_two_minus, dim3(eblocks(N * 2 * h)), dim3(256), 0, s, T3, 2 * h,
+                           N * 2 * h, p);
+        // I_(i+1)(j) = g0(j) (2 - e(j)) mod x^(2h)
+        for (int64_t j = 0; j < N; ++j) {
+            MulDesc &m = d[(size_t)j];
+            m.f = j * h; m.g = j * 2 * h; m.h = j * 2 * h;
+            m.lf = h; m.lg = 2 * h; m.lh = 2 * h; m.logn = i + 2;
+        }
+        e = launch_list_mul(d.data(), N, T1, T3, inv_at(i + 1), fam, s);
+    }
+    if (e != hipSuccess) return e;
+    // ---- remainder tree: R_k = F, then levels k-1 .. L0
+    e = hipMemcpyAsync(R0, F, sizeof(u32) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    u32 *A = R0, *Rn = R1;
+    for (int i = k - 1; i >= L0 && e == hipSuccess; --i) {
+        const int h = 1 << i;
+        const int64_t N = n >> i;                            // nodes of level i
+        hipLaunchKernelGGL(k_div_rev_top, dim3(eblocks(N * h)), dim3(256), 0, s, A, h, N, T1);
+        // qrev(j) = revtop(j) I_i(j) mod x^h
+        d.assign((size_t)N, MulDesc{});
+        for (int64_t j = 0; j < N; ++j) {
+            MulDesc &m = d[(size_t)j];
+            m.f = j * h; m.g = j * h; m.h = j * h;
+            m.lf = m.lg = h; m.lh = h; m.logn = i + 1;
+        }
+        e = launch_list_mul(d.data(), N, T1, inv_at(i), T2, fam, s);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(k_reverse, dim3(eblocks(N * h)), dim3(256), 0, s, T2, h, N, T1);
+        // qm(j) = q(j) M_i(j) mod x^h (the leading x^h of M vanishes mod x^h)
+        const int64_t moff = tree_level_offset(k, i);
+        for (int64_t j = 0; j < N; ++j) {
+            MulDesc &m = d[(size_t)j];
+            m.f = j * h; m.g = moff + j * (h + 1); m.h = j * h;
+            m.lf = h; m.lg = h; m.lh = h; m.logn = i + 1;
+        }
+        e = launch_list_mul(d.data(), N, T1, tree, T2, fam, s);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(k_div_finish, dim3(eblocks(N * h)), dim3(256), 0, s, A, T2, h, N, p,
+                           Rn);
+        e = hipGetLastError();
+        u32 *t = A; A = Rn; Rn = t;
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_horner, dim3(eblocks(n)), dim3(256), 0, s, A, 1 << L0, pts, n, p,
+                       values);
+    return hipGetLastError();
+}
+
+}  // namespace cm

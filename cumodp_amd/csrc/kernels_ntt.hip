@@ -30,8 +30,11 @@ CM_DEV int xcd_remap(int b, int nb) {
 // ---------------------------------------------------------------------------
 // DESC: descriptors from `descs` (or the by-value `one` when descs is null);
 // otherwise row d of F, G, H at offset d * L, full length.  H may alias F.
+// Held to 128 VGPRs so that the 2^12- and 2^13-point tiles keep as many
+// workgroups per CU as their LDS allows.
 template <int LOGL, int C, bool DESC>
-__global__ __launch_bounds__((Tile<LOGL, C, false>::T)) void k_mul_lds(
+__global__ __launch_bounds__((Tile<LOGL, C, false>::T))
+__attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
     const u32 *F, const u32 *__restrict__ G, u32 *H, const MulDesc *__restrict__ descs,
     int ndesc, MulDesc one, const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md,
     u32 sc) {
@@ -178,9 +181,11 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
 
 // Inverse column transforms (DIT, bit-reversed rows in, natural rows out)
 // with a truncating store; the inverse outer twiddle was applied by the row
-// pass.  grid.y: arrays.
+// pass.  grid.y: arrays.  Held to 64 VGPRs: at 65 only seven waves fit per
+// SIMD, i.e. one 16-wave workgroup per CU instead of two.
 template <int LOGL1, int C>
-__global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_inv(
+__global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
+__attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
     ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];

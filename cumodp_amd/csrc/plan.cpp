@@ -263,6 +263,26 @@ hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s)
     return hipEventRecord(st.done, s);
 }
 
+namespace {
+std::map<hipStream_t, AuxStreams *> g_aux;   // guarded by g_ws_mutex
+}  // namespace
+
+AuxStreams *aux_streams(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    auto it = g_aux.find(s);
+    if (it != g_aux.end()) return it->second;
+    if (stream_capturing(s)) return nullptr;   // created eagerly, never inside a capture
+    AuxStreams *a = new AuxStreams();
+    bool ok = hipEventCreateWithFlags(&a->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < kAuxStreams && ok; ++i) {
+        ok = hipStreamCreate(&a->s[i]) == hipSuccess &&
+             hipEventCreateWithFlags(&a->join[i], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) return nullptr;   // leaks the partial set; the caller runs serially
+    g_aux[s] = a;
+    return a;
+}
+
 std::vector<void *> take_captured_buffers(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_ws_mutex);
     std::vector<void *> v;
@@ -275,6 +295,25 @@ std::vector<void *> take_captured_buffers(hipStream_t s) {
 }
 
 void release_workspace(hipStream_t s) {
+    AuxStreams *a = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mutex);
+        auto it = g_aux.find(s);
+        if (it != g_aux.end()) {
+            a = it->second;
+            g_aux.erase(it);
+        }
+    }
+    if (a) {
+        for (int i = 0; i < kAuxStreams; ++i) {
+            (void)hipStreamSynchronize(a->s[i]);
+            release_workspace(a->s[i]);
+            (void)hipStreamDestroy(a->s[i]);
+            (void)hipEventDestroy(a->join[i]);
+        }
+        (void)hipEventDestroy(a->fork);
+        delete a;
+    }
     std::lock_guard<std::mutex> lk(g_ws_mutex);
     for (auto it = g_ws.begin(); it != g_ws.end();) {
         if (it->first.second == s) {

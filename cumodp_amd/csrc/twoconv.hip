@@ -74,6 +74,36 @@ static bool tc_choose(int bits, int64_t na, int64_t nb, TcPlan &pl) {
     return true;
 }
 
+// Plan shared by a class of batched products with y size 2^ey: the cheapest
+// (np, K) whose widest admissible limb M_max (the CRT bound with every one of
+// the K limbs nonzero and n_min = 2^(ey-1), the class's worst case) covers
+// `bits`, with M = M_max.  Products of different bit widths then share a
+// plan, and so a launch sequence, whenever one (np, K) serves them.
+static bool tc_class_plan(int bits, int ey, TcPlan &pl) {
+    const double lgP[2] = {std::log2(2013265921.0) + std::log2(2113929217.0),
+                           std::log2(2013265921.0) + std::log2(2113929217.0) +
+                               std::log2(1811939329.0)};
+    int best = 0;
+    for (int np = 2; np <= 3; ++np) {
+        for (int K = 1; K <= 512; K *= 2) {
+            int M = 30;
+            while (M >= 1 && std::log2((double)K) + (ey - 1) + 2.0 * M >= lgP[np - 2] - 1.01) --M;
+            if (M < 1 || (int64_t)K * M < bits) continue;
+            const int cost = np * K;
+            if (!best || cost < best) {
+                best = cost;
+                pl.np = np; pl.K = K; pl.M = M;
+            }
+            break;
+        }
+    }
+    if (!best) return false;
+    pl.lk2 = 0;
+    while ((1 << pl.lk2) < 2 * pl.K) ++pl.lk2;
+    pl.ey = ey;
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -327,10 +357,9 @@ hipError_t launch_two_convolution(const u64 *a, int64_t na, int wa, const u64 *b
     return tc_run_class(pl, a, b, c, nullptr, one, 1, na + nb - 1, ws, s);
 }
 
-// Batch of products: items are grouped by plan (chosen for the worst case of
-// their y size, so one plan serves the class) and each class runs as one
-// sequence of batched launches, in chunks whose scratch stays under
-// kTcChunkBytes.
+// Batch of products: items are grouped by class plan (tc_class_plan: one plan
+// per y size and (np, K)) and each class runs as one sequence of batched
+// launches, in chunks whose scratch stays under kTcChunkBytes.
 static constexpr size_t kTcChunkBytes = (size_t)1 << 30;
 
 hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const u64 *A,
@@ -346,14 +375,11 @@ hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const
         int ey = 0;
         while (((int64_t)1 << ey) < it.na + it.nb - 1) ++ey;
         if (ey < 4) ey = 4;
-        // worst case of the class: n_min up to 2^(ey-1)
-        const int64_t nm = (int64_t)1 << (ey > 1 ? ey - 1 : 0);
         TcPlan pl;
-        if (!tc_choose(it.bits, nm, nm, pl) || pl.lk2 > kMaxLdsLog) {
+        if (!tc_class_plan(it.bits, ey, pl) || pl.lk2 > kMaxLdsLog) {
             *err = 1;
             return hipSuccess;
         }
-        pl.ey = ey;
         if (pl.ey > kMaxLog) {
             *err = 1;
             return hipSuccess;
@@ -379,8 +405,23 @@ hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const
     if (!dd) return hipErrorOutOfMemory;
     hipError_t e = stage_upload(hd.data(), sizeof(TcDesc) * hd.size(), dd, s);
     if (e != hipSuccess) return e;
+    // Classes are independent: with more than one they run on the caller
+    // stream's auxiliary streams (forked after the descriptor upload, joined
+    // back at the end), so one class's launch gaps overlap another's work.
+    // Each auxiliary stream has its own scratch.
+    const int nclass = (int)classes.size();
+    AuxStreams *aux = nclass > 1 ? aux_streams(s) : nullptr;
+    const int nuse = aux ? std::min(nclass, (int)kAuxStreams) : 0;
+    if (aux) {
+        e = hipEventRecord(aux->fork, s);
+        for (int i = 0; i < nuse && e == hipSuccess; ++i)
+            e = hipStreamWaitEvent(aux->s[i], aux->fork, 0);
+        if (e != hipSuccess) return e;
+    }
     size_t first = 0;
-    for (auto &c : classes) {
+    for (int ci = 0; ci < nclass; ++ci) {
+        const auto &c = classes[(size_t)ci];
+        const hipStream_t cs = aux ? aux->s[ci % nuse] : s;
         const int K2 = 1 << c.pl.lk2;
         const int64_t Ly = (int64_t)1 << c.pl.ey;
         const size_t item_bytes = sizeof(u32) * (size_t)2 * K2 * Ly * c.pl.np;
@@ -389,7 +430,7 @@ hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const
         const int64_t n = (int64_t)c.idx.size();
         for (int64_t k0 = 0; k0 < n; k0 += chunk) {
             const int R = (int)std::min<int64_t>(chunk, n - k0);
-            u32 *ws = (u32 *)workspace(10, item_bytes * (size_t)R, s);
+            u32 *ws = (u32 *)workspace(10, item_bytes * (size_t)R, cs);
             if (!ws) return hipErrorOutOfMemory;
             int64_t ymax = 1;
             for (int r = 0; r < R; ++r) {
@@ -397,10 +438,15 @@ hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const
                 ymax = std::max<int64_t>(ymax, it.na + it.nb - 1);
             }
             TcDesc none{};
-            e = tc_run_class(c.pl, A, B, C, dd + first + k0, none, R, ymax, ws, s);
+            e = tc_run_class(c.pl, A, B, C, dd + first + k0, none, R, ymax, ws, cs);
             if (e != hipSuccess) return e;
         }
         first += (size_t)n;
+    }
+    for (int i = 0; i < nuse; ++i) {
+        e = hipEventRecord(aux->join[i], aux->s[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, aux->join[i], 0);
+        if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }

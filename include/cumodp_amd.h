@@ -147,6 +147,18 @@ CUMODP_API cumodp_err cumodp_graph_begin(void *stream);
 CUMODP_API void *cumodp_graph_end(void *stream);
 CUMODP_API cumodp_err cumodp_graph_launch(void *graph, void *stream);
 CUMODP_API cumodp_err cumodp_graph_destroy(void *graph);
+/* ---- fast multipoint evaluation --------------------------------------------
+ * fastEvaluation (src/fastPolyEvaluation.cu:1371-1400, with subInvTree :666
+ * and the remainder descent :1088-1370): values[j] = F(points[j]) mod p for
+ * the 2^k points, F given by its 2^k coefficients (degree < 2^k).  Built on
+ * the subproduct tree, a subinverse tree (inverses of the reversed nodes by
+ * Newton iteration) and a remainder tree whose divisions are batched
+ * products.  The reference returns its internal per-level buffers
+ * (struct PolyEvalSteps); these return the values. */
+CUMODP_API cumodp_err cumodp_fast_evaluation_dev(const sfixn *F, const sfixn *points, int k,
+                                                 sfixn p, sfixn *values, void *stream);
+CUMODP_API cumodp_err cumodp_fast_evaluation(const sfixn *F, const sfixn *points, int k, sfixn p,
+                                             sfixn *values);
 /* ---- tuning: four-step split ----------------------------------------------
  * Transforms of 2^k points above 2^13 run as 2^(k-k2) columns x 2^k2 rows;
  * this chooses k2 (12, 13 or 14) for one k.  The tuner (cumodp_amd/autotune.py)

@@ -64,6 +64,7 @@ def lib():
                                                C.c_int, u64p, C.c_int]
         L.orc_two_convolution_zmul.restype = C.c_int
         L.orc_subproduct_level.argtypes = [i32p, C.c_int64, C.c_int, s, i32p]
+        L.orc_poly_eval.argtypes = [i32p, C.c_int64, i32p, C.c_int64, s, i32p]
         L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_splitmix64.restype = C.c_uint64
         _LIB = L
@@ -237,3 +238,12 @@ def subproduct_level(points, lev, p):
     out = np.zeros((n >> lev) * ((1 << lev) + 1), np.int32)
     lib().orc_subproduct_level(pts, n, lev, p, out)
     return out.reshape(n >> lev, (1 << lev) + 1)
+
+
+# ---- multipoint evaluation ---------------------------------------------------
+def poly_eval(F, X, p):
+    """F(X[j]) mod p by Horner's rule."""
+    F, X = _i32(F), _i32(X)
+    out = np.zeros(len(X), np.int32)
+    lib().orc_poly_eval(F, len(F), X, len(X), p, out)
+    return out

@@ -664,6 +664,21 @@ void orc_subproduct_level(const sfixn *pts, int64_t npts, int lev, sfixn p,
     free(cur);
 }
 
+/* ------------------------------------------------------------------------ */
+/* multipoint evaluation                                                    */
+/* ------------------------------------------------------------------------ */
+void orc_poly_eval(const sfixn *F, int64_t n, const sfixn *X, int64_t m, sfixn p,
+                   sfixn *out)
+{
+    for (int64_t j = 0; j < m; ++j) {
+        const uint64_t x = (uint64_t)(uint32_t)X[j] % (uint64_t)p;
+        uint64_t acc = 0;
+        for (int64_t t = n - 1; t >= 0; --t)
+            acc = (acc * x + (uint64_t)(uint32_t)F[t]) % (uint64_t)p;
+        out[j] = (sfixn)acc;
+    }
+}
+
 uint64_t orc_splitmix64(uint64_t *state)
 {
     uint64_t z = (*state += 0x9E3779B97F4A7C15ULL);

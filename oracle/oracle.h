@@ -25,6 +25,9 @@
  *   orc_scale_x                          -- reference gtest KATs.
  *   orc_two_convolution_zmul             -- equal to orc_zmul_* on KATs,
  *                                           fixtures and random inputs.
+ *   orc_poly_eval                        -- the definition (Horner); checked
+ *                                           against direct power sums and the
+ *                                           subproduct tree's roots.
  *   orc_subproduct_level                 -- evaluation of prod (x - x_i) at
  *                                           random points.
  * No function is left "parity unpinned".
@@ -118,6 +121,13 @@ int orc_two_convolution_zmul(const uint64_t *a, int64_t na, int wa,
  * blocks of 2^lev consecutive points, leading coefficient included. */
 void orc_subproduct_level(const sfixn *pts, int64_t npts, int lev, sfixn p,
                           sfixn *out);
+
+/* ---- multipoint evaluation (src/fastPolyEvaluation.cu:1371-1400) ---- */
+/* out[j] = F(X[j]) mod p by Horner's rule, F of n coefficients: the
+ * definition the reference's fast evaluation computes (its own check
+ * compares against Maple's values). */
+void orc_poly_eval(const sfixn *F, int64_t n, const sfixn *X, int64_t m, sfixn p,
+                   sfixn *out);
 
 /* SplitMix64 (Steele, Lea, Flood 2014): next output, advancing *state */
 uint64_t orc_splitmix64(uint64_t *state);

@@ -300,3 +300,20 @@ def test_splitmix_matches_numpy_stream(orc):
     st = ctypes.c_uint64(12345)
     c = [orc.lib().orc_splitmix64(ctypes.byref(st)) for _ in range(8)]
     assert [int(x) for x in wl.SplitMix64(12345).next(8)] == c
+
+
+def test_poly_eval_closed_forms(orc):
+    """Horner evaluation against the definition and the tree's roots."""
+    p = 469762049
+    rng = np.random.default_rng(7)
+    F = rng.integers(0, p, 40).astype(np.int32)
+    X = rng.integers(0, p, 25).astype(np.int32)
+    want = [sum(int(F[t]) * pow(int(x), t, p) for t in range(len(F))) % p for x in X]
+    assert [int(v) for v in orc.poly_eval(F, X, p)] == want
+    # the root of the subproduct tree vanishes at every point
+    pts = rng.integers(0, p, 16).astype(np.int32)
+    root = orc.subproduct_level(pts, 4, p)[0]
+    assert (orc.poly_eval(root, pts, p) == 0).all()
+    # constant and zero polynomials
+    assert (orc.poly_eval(np.array([5], np.int32), X, p) == 5).all()
+    assert (orc.poly_eval(np.zeros(3, np.int32), X, p) == 0).all()
