@@ -137,15 +137,23 @@ struct ColSet {
     int64_t in_stride, in_len, out_stride, out_len;
 };
 
+// Column tiles move through memory either one word per lane or, when VEC
+// (C a multiple of 4, 16-byte aligned arrays and strides), four consecutive
+// columns of a row per lane: a quarter of the memory instructions, each 16
+// bytes.  The tile's LDS layout (element (i, c) at cm_pad(i C + c)) is the
+// same either way.
+
 // Forward column transforms (DIF, bit-reversed rows), followed by the outer
 // twiddle w_n^(i2 * k1) when OUTER (the multiply applies it in the row pass
 // instead).  grid.x: column tiles, grid.y: arrays (the first cnt0 from set
 // s0, the rest from s1).
-template <int LOGL1, int C, bool OUTER>
-__global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
+template <int LOGL1, int C, bool OUTER, bool VEC>
+__global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
+__attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
     ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv,
     const u32 *__restrict__ olo, const u32 *__restrict__ ohi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
+    static_assert(!VEC || (C % 4 == 0 && !OUTER), "vector column access");
     __shared__ u32 lds[TL::LDS_WORDS];
     const int y = blockIdx.y;
     const ColSet &S = y < cnt0 ? s0 : s1;
@@ -155,27 +163,66 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
     const int64_t n2 = (int64_t)1 << logn2;
     const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
     constexpr int PER = TL::WORDS / TL::T;
-    u32 xa[PER];
+    if constexpr (VEC) {
+        constexpr int PV = PER / 4;
+        uint4 xa[PV];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
-        const int64_t e = i * n2 + c0 + c;
-        xa[r] = e < S.in_len ? X[e] : 0u;
+        for (int r = 0; r < PV; ++r) {
+            const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
+            const int64_t e = i * n2 + c0 + c;
+            if (e + 3 < S.in_len) {
+                xa[r] = *(const uint4 *)(X + e);
+            } else {
+                xa[r].x = e < S.in_len ? X[e] : 0u;
+                xa[r].y = e + 1 < S.in_len ? X[e + 1] : 0u;
+                xa[r].z = e + 2 < S.in_len ? X[e + 2] : 0u;
+                xa[r].w = 0u;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PV; ++r) {
+            const int idx = 4 * (threadIdx.x + r * TL::T);
+            lds[cm_pad(idx)] = xa[r].x;
+            lds[cm_pad(idx + 1)] = xa[r].y;
+            lds[cm_pad(idx + 2)] = xa[r].z;
+            lds[cm_pad(idx + 3)] = xa[r].w;
+        }
+    } else {
+        u32 xa[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
+            const int64_t e = i * n2 + c0 + c;
+            xa[r] = e < S.in_len ? X[e] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     }
-#pragma unroll
-    for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
     cm_tile_dif<TL>(lds, lv, md);
-    constexpr int LOM = (1 << kOuterLoBits) - 1;
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
-        const int i = idx / C, c = idx % C;
-        u32 v = lds[cm_pad(idx)];
-        if (OUTER) {
-            const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
-            v = cm_mul_tw(cm_mont_lazy(v, olo[e & LOM], md.p, md.np), ohi[e >> kOuterLoBits],
-                          md.p, md.np);
+    if constexpr (VEC) {
+#pragma unroll
+        for (int r = 0; r < PER / 4; ++r) {
+            const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
+            uint4 v;
+            v.x = lds[cm_pad(idx)];
+            v.y = lds[cm_pad(idx + 1)];
+            v.z = lds[cm_pad(idx + 2)];
+            v.w = lds[cm_pad(idx + 3)];
+            *(uint4 *)(Y + i * n2 + c0 + c) = v;
         }
-        Y[i * n2 + c0 + c] = v;
+    } else {
+        constexpr int LOM = (1 << kOuterLoBits) - 1;
+        for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
+            const int i = idx / C, c = idx % C;
+            u32 v = lds[cm_pad(idx)];
+            if (OUTER) {
+                const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
+                v = cm_mul_tw(cm_mont_lazy(v, olo[e & LOM], md.p, md.np),
+                              ohi[e >> kOuterLoBits], md.p, md.np);
+            }
+            Y[i * n2 + c0 + c] = v;
+        }
     }
 }
 
@@ -183,11 +230,12 @@ __global__ __launch_bounds__((Tile<LOGL1, C, true>::T)) void k_col_fwd(
 // with a truncating store; the inverse outer twiddle was applied by the row
 // pass.  grid.y: arrays.  Held to 64 VGPRs: at 65 only seven waves fit per
 // SIMD, i.e. one 16-wave workgroup per CU instead of two.
-template <int LOGL1, int C>
+template <int LOGL1, int C, bool VEC>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
 __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
     ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
+    static_assert(!VEC || C % 4 == 0, "vector column access");
     __shared__ u32 lds[TL::LDS_WORDS];
     const int64_t a = blockIdx.y;
     const u32 *X = S.in + a * S.in_stride;
@@ -195,20 +243,58 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
     const int64_t n2 = (int64_t)1 << logn2;
     const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
     constexpr int PER = TL::WORDS / TL::T;
-    u32 xa[PER];
+    if constexpr (VEC) {
+        constexpr int PV = PER / 4;
+        uint4 xa[PV];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
-        xa[r] = X[i * n2 + c0 + c];
+        for (int r = 0; r < PV; ++r) {
+            const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
+            xa[r] = *(const uint4 *)(X + i * n2 + c0 + c);
+        }
+#pragma unroll
+        for (int r = 0; r < PV; ++r) {
+            const int idx = 4 * (threadIdx.x + r * TL::T);
+            lds[cm_pad(idx)] = xa[r].x;
+            lds[cm_pad(idx + 1)] = xa[r].y;
+            lds[cm_pad(idx + 2)] = xa[r].z;
+            lds[cm_pad(idx + 3)] = xa[r].w;
+        }
+    } else {
+        u32 xa[PER];
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
+            xa[r] = X[i * n2 + c0 + c];
+        }
+#pragma unroll
+        for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     }
-#pragma unroll
-    for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
     cm_tile_dit<TL>(lds, lvi, md);
-    for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
-        const int i = idx / C, c = idx % C;
-        const int64_t e = i * n2 + c0 + c;
-        if (e < S.out_len) Y[e] = lds[cm_pad(idx)];
+    if constexpr (VEC) {
+#pragma unroll
+        for (int r = 0; r < PER / 4; ++r) {
+            const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
+            const int64_t e = i * n2 + c0 + c;
+            uint4 v;
+            v.x = lds[cm_pad(idx)];
+            v.y = lds[cm_pad(idx + 1)];
+            v.z = lds[cm_pad(idx + 2)];
+            v.w = lds[cm_pad(idx + 3)];
+            if (e + 3 < S.out_len) {
+                *(uint4 *)(Y + e) = v;
+            } else {
+                if (e < S.out_len) Y[e] = v.x;
+                if (e + 1 < S.out_len) Y[e + 1] = v.y;
+                if (e + 2 < S.out_len) Y[e + 2] = v.z;
+            }
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
+            const int i = idx / C, c = idx % C;
+            const int64_t e = i * n2 + c0 + c;
+            if (e < S.out_len) Y[e] = lds[cm_pad(idx)];
+        }
     }
 }
 
@@ -448,6 +534,12 @@ static constexpr int col_tile() {
     return (WORDS >> LOGL1) > 4096 ? 4096 : (WORDS >> LOGL1);
 }
 
+// 16-byte vector access to both arrays of a column set
+static bool vec_ok(const ColSet &S) {
+    return ((uintptr_t)S.in & 15) == 0 && ((uintptr_t)S.out & 15) == 0 && (S.in_stride & 3) == 0 &&
+           (S.out_stride & 3) == 0;
+}
+
 // Words per column tile of the multiply's four-step column passes: 16384
 // (two 1024-thread workgroups per CU) or 8192 (four 512-thread workgroups,
 // half the row segment per tile); a tuned choice.
@@ -470,11 +562,19 @@ static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColFwd, s);
     if (olo)
-        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0, s,
-                           s0, s1, cnt0, logn2, lv, olo, ohi, md);
-    else
-        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false>), dim3(nb, cnt0 + cnt1), dim3(TL::T), 0,
-                           s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true, false>), dim3(nb, cnt0 + cnt1),
+                           dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+    else if constexpr (C % 4 == 0) {
+        if (vec_ok(s0) && (cnt1 == 0 || vec_ok(s1)))
+            hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, true>), dim3(nb, cnt0 + cnt1),
+                               dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+        else
+            hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, false>), dim3(nb, cnt0 + cnt1),
+                               dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+    } else {
+        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, false>), dim3(nb, cnt0 + cnt1),
+                           dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+    }
     return hipGetLastError();
 }
 
@@ -496,7 +596,14 @@ static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *f
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColInv, s);
-    hipLaunchKernelGGL((k_col_inv<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S, logn2,
+    if constexpr (C % 4 == 0) {
+        if (vec_ok(S)) {
+            hipLaunchKernelGGL((k_col_inv<LOGL1, C, true>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
+                               logn2, fam->lvi, mod_of(fam));
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_col_inv<LOGL1, C, false>), dim3(nb, cnt), dim3(TL::T), 0, s, S, logn2,
                        fam->lvi, mod_of(fam));
     return hipGetLastError();
 }
