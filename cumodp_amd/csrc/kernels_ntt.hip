@@ -140,7 +140,7 @@ struct ColSet {
 // Column tiles move through memory either one word per lane or, when VEC
 // (C a multiple of 4, 16-byte aligned arrays and strides), four consecutive
 // columns of a row per lane: a quarter of the memory instructions, each 16
-// bytes.  The tile's LDS layout (element (i, c) at cm_pad(i C + c)) is the
+// bytes.  The tile's LDS layout (element (i, c) at TL::phys(i C + c)) is the
 // same either way.
 
 // Forward column transforms (DIF, bit-reversed rows), followed by the outer
@@ -182,10 +182,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
 #pragma unroll
         for (int r = 0; r < PV; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T);
-            lds[cm_pad(idx)] = xa[r].x;
-            lds[cm_pad(idx + 1)] = xa[r].y;
-            lds[cm_pad(idx + 2)] = xa[r].z;
-            lds[cm_pad(idx + 3)] = xa[r].w;
+            lds[TL::phys(idx)] = xa[r].x;
+            lds[TL::phys(idx + 1)] = xa[r].y;
+            lds[TL::phys(idx + 2)] = xa[r].z;
+            lds[TL::phys(idx + 3)] = xa[r].w;
         }
     } else {
         u32 xa[PER];
@@ -196,7 +196,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
             xa[r] = e < S.in_len ? X[e] : 0u;
         }
 #pragma unroll
-        for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
+        for (int r = 0; r < PER; ++r) lds[TL::phys(threadIdx.x + r * TL::T)] = xa[r];
     }
     __syncthreads();
     cm_tile_dif<TL>(lds, lv, md);
@@ -205,17 +205,17 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
         for (int r = 0; r < PER / 4; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
             uint4 v;
-            v.x = lds[cm_pad(idx)];
-            v.y = lds[cm_pad(idx + 1)];
-            v.z = lds[cm_pad(idx + 2)];
-            v.w = lds[cm_pad(idx + 3)];
+            v.x = lds[TL::phys(idx)];
+            v.y = lds[TL::phys(idx + 1)];
+            v.z = lds[TL::phys(idx + 2)];
+            v.w = lds[TL::phys(idx + 3)];
             *(uint4 *)(Y + i * n2 + c0 + c) = v;
         }
     } else {
         constexpr int LOM = (1 << kOuterLoBits) - 1;
         for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
             const int i = idx / C, c = idx % C;
-            u32 v = lds[cm_pad(idx)];
+            u32 v = lds[TL::phys(idx)];
             if (OUTER) {
                 const u32 e = (u32)(c0 + c) * cm_brev(i, LOGL1);
                 v = cm_mul_tw(cm_mont_lazy(v, olo[e & LOM], md.p, md.np),
@@ -254,10 +254,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
 #pragma unroll
         for (int r = 0; r < PV; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T);
-            lds[cm_pad(idx)] = xa[r].x;
-            lds[cm_pad(idx + 1)] = xa[r].y;
-            lds[cm_pad(idx + 2)] = xa[r].z;
-            lds[cm_pad(idx + 3)] = xa[r].w;
+            lds[TL::phys(idx)] = xa[r].x;
+            lds[TL::phys(idx + 1)] = xa[r].y;
+            lds[TL::phys(idx + 2)] = xa[r].z;
+            lds[TL::phys(idx + 3)] = xa[r].w;
         }
     } else {
         u32 xa[PER];
@@ -267,7 +267,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
             xa[r] = X[i * n2 + c0 + c];
         }
 #pragma unroll
-        for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
+        for (int r = 0; r < PER; ++r) lds[TL::phys(threadIdx.x + r * TL::T)] = xa[r];
     }
     __syncthreads();
     cm_tile_dit<TL>(lds, lvi, md);
@@ -277,10 +277,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
             const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
             const int64_t e = i * n2 + c0 + c;
             uint4 v;
-            v.x = lds[cm_pad(idx)];
-            v.y = lds[cm_pad(idx + 1)];
-            v.z = lds[cm_pad(idx + 2)];
-            v.w = lds[cm_pad(idx + 3)];
+            v.x = lds[TL::phys(idx)];
+            v.y = lds[TL::phys(idx + 1)];
+            v.z = lds[TL::phys(idx + 2)];
+            v.w = lds[TL::phys(idx + 3)];
             if (e + 3 < S.out_len) {
                 *(uint4 *)(Y + e) = v;
             } else {
@@ -293,7 +293,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
         for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
             const int i = idx / C, c = idx % C;
             const int64_t e = i * n2 + c0 + c;
-            if (e < S.out_len) Y[e] = lds[cm_pad(idx)];
+            if (e < S.out_len) Y[e] = lds[TL::phys(idx)];
         }
     }
 }
@@ -332,8 +332,8 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     for (int m = 0; m < 16; ++m) {
         const int i = t + T * m;
         const u32 c = rc[k1 * 16 + m];
-        A[cm_pad(i)] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md.p, md.np);
-        B[cm_pad(i)] = cm_mul_tw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md.p, md.np);
+        A[TL::phys(i)] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md.p, md.np);
+        B[TL::phys(i)] = cm_mul_tw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md.p, md.np);
     }
     __syncthreads();
     cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
@@ -342,7 +342,7 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     for (int m = 0; m < 16; ++m) {
         const int i = t + T * m;
         const u32 c = rci[k1 * 16 + m];
-        Fr[i] = cm_mul_tw(cm_mont_lazy(A[cm_pad(i)], c, md.p, md.np), bi, md.p, md.np);
+        Fr[i] = cm_mul_tw(cm_mont_lazy(A[TL::phys(i)], c, md.p, md.np), bi, md.p, md.np);
     }
 }
 
@@ -399,13 +399,13 @@ __global__ __launch_bounds__((Tile<LOGM, C, true>::T)) void k_col_ntt(
         xa[r] = (c0 + c < ncols) ? X[i * ncols + c0 + c] : 0u;
     }
 #pragma unroll
-    for (int r = 0; r < PER; ++r) lds[cm_pad(threadIdx.x + r * TL::T)] = xa[r];
+    for (int r = 0; r < PER; ++r) lds[TL::phys(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
     cm_tile_dif<TL>(lds, lv, md);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;
         if (c0 + c >= ncols) continue;
-        u32 v = lds[cm_pad((int)cm_brev(i, LOGM) * C + c)];
+        u32 v = lds[TL::phys((int)cm_brev(i, LOGM) * C + c)];
         if (sc) v = cm_mul_tw(v, sc, md.p, md.np);
         X[i * ncols + c0 + c] = v;
     }

@@ -78,7 +78,17 @@ struct Tile {
     static constexpr int C = C_;
     static constexpr bool COLS = COLS_;
     static constexpr int WORDS = C_ * L;
-    static constexpr int LDS_WORDS = WORDS + (WORDS >> 4);
+    // LDS placement of tile-linear index x.  Rows of 2^13 words and more XOR
+    // the three bits above each 64-word block into the low six
+    // (x ^ 9 ((x >> 6) & 7)): every round's wave access and the linear load
+    // and store phases then hit 64 distinct banks (at worst two per bank on a
+    // 32-bank LDS), with no padding; the row pass's bank-conflict cycles drop
+    // eightfold.  Other tiles keep one pad word per 16: for the column tiles
+    // the swizzle removes conflicts too, but its address arithmetic does not
+    // fold into LDS instruction offsets and pushes them past 64 VGPRs.
+    static constexpr bool SWZ = !COLS_ && LOGL_ >= 13;
+    static constexpr int LDS_WORDS = SWZ ? WORDS : WORDS + (WORDS >> 4);
+    static CM_DEV int phys(int x) { return SWZ ? (x ^ (((x >> 6) & 7) * 9)) : cm_pad(x); }
     static constexpr int T = T_;                   // threads per workgroup
     static constexpr int NR = (LOGL_ + 3) / 4;     // rounds
     static constexpr int rsz(int j) { return LOGL_ / NR + (j < LOGL_ % NR ? 1 : 0); }
@@ -88,7 +98,7 @@ struct Tile {
         return s;
     }
     // LDS address of element i of vector c
-    static CM_DEV int addr(int c, int i) { return cm_pad(COLS_ ? i * C_ + c : c * L + i); }
+    static CM_DEV int addr(int c, int i) { return phys(COLS_ ? i * C_ + c : c * L + i); }
     // group g of a round with 2^(LOGL-R) groups per vector -> (vector, index)
     template <int R>
     static CM_DEV void split(int g, int &c, int &gi) {

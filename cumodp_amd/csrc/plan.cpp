@@ -223,7 +223,15 @@ struct Staging {
     size_t cap = 0;
     hipEvent_t done = nullptr;
 };
-std::map<hipStream_t, Staging> g_staging;   // guarded by g_ws_mutex
+// A ring of staging buffers per stream: a buffer is rewritten only once the
+// copy that last read it has run, so the host can queue several calls ahead
+// of the device without waiting.
+constexpr int kStagingRing = 8;
+struct StagingRing {
+    Staging b[kStagingRing];
+    int next = 0;
+};
+std::map<hipStream_t, StagingRing> g_staging;   // guarded by g_ws_mutex
 // pinned buffers whose copies were recorded into a graph being captured on a
 // stream; they belong to that graph once the capture ends
 std::map<hipStream_t, std::vector<void *>> g_captured;
@@ -241,7 +249,22 @@ hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s)
         g_captured[s].push_back(h);
         return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s);
     }
-    Staging &st = g_staging[s];
+    // prefer a buffer that is already large enough and whose last copy has
+    // run; else an unused slot; else wait for the oldest
+    StagingRing &ring = g_staging[s];
+    int pick = -1;
+    for (int i = 0; i < kStagingRing && pick < 0; ++i) {
+        const int j = (ring.next + i) % kStagingRing;
+        Staging &b = ring.b[j];
+        if (b.done && b.cap >= bytes && hipEventQuery(b.done) == hipSuccess) pick = j;
+    }
+    for (int i = 0; i < kStagingRing && pick < 0; ++i) {
+        const int j = (ring.next + i) % kStagingRing;
+        if (!ring.b[j].done) pick = j;
+    }
+    if (pick < 0) pick = ring.next;
+    ring.next = (pick + 1) % kStagingRing;
+    Staging &st = ring.b[pick];
     if (!st.done) {
         e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
         if (e != hipSuccess) return e;
@@ -325,8 +348,10 @@ void release_workspace(hipStream_t s) {
     }
     auto st = g_staging.find(s);
     if (st != g_staging.end()) {
-        if (st->second.done) (void)hipEventDestroy(st->second.done);
-        if (st->second.host) (void)hipHostFree(st->second.host);
+        for (Staging &b : st->second.b) {
+            if (b.done) (void)hipEventDestroy(b.done);
+            if (b.host) (void)hipHostFree(b.host);
+        }
         g_staging.erase(st);
     }
 }
