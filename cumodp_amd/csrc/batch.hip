@@ -156,7 +156,21 @@ hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, co
                      [&](int64_t a, int64_t b) { return descs[a].logn < descs[b].logn; });
     std::vector<MulDesc> sorted((size_t)count);
     for (int64_t i = 0; i < count; ++i) sorted[(size_t)i] = descs[order[(size_t)i]];
-    const int mv = g_mix_variant;
+    // The tuned tile suits large batches.  A batch whose small products fill
+    // fewer than two tiles per CU gets smaller tiles (more workgroups), down
+    // to 2048 words; tile sizes below the largest small product are skipped.
+    int mv = g_mix_variant;
+    {
+        int64_t small_words = 0;
+        int max_small = 0;
+        for (int64_t q = 0; q < count && sorted[(size_t)q].logn <= 13; ++q) {
+            small_words += (int64_t)1 << sorted[(size_t)q].logn;
+            max_small = std::max(max_small, (int)sorted[(size_t)q].logn);
+        }
+        while (mv > 0 && small_words / kMixWordsOf[mv] < 2 * 256 &&
+               (1 << max_small) <= kMixWordsOf[mv - 1])
+            --mv;
+    }
     const int mix_words = kMixWordsOf[mv];
     std::vector<TileDesc> tiles;
     int64_t i = 0;

@@ -17,10 +17,9 @@
 //     That is two batched products per level;
 //   * at level L0 each remainder (2^L0 coefficients) is evaluated at its
 //     node's 2^L0 points by Horner's rule.
-// Every batched product of a level is one launch_list_mul call (mixed-size
-// LDS kernel for small nodes, four-step for large ones).
-#include <vector>
-
+// Every batched product of a level is one launch_mul_strided call: the
+// level's products share one shape, so they need no descriptors (one LDS
+// multiply per workgroup up to 2^13 points, batched four-step above).
 #include "internal.h"
 
 namespace cm {
@@ -146,39 +145,24 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
                            n >> L0, p, inv_at(L0));
         e = hipGetLastError();
     }
-    std::vector<MulDesc> d;
     for (int i = L0; i + 1 <= k - 1 && e == hipSuccess; ++i) {
         const int h = 1 << i;
-        const int64_t N = n >> (i + 1);                      // nodes of level i + 1
+        const int N = (int)(n >> (i + 1));                   // nodes of level i + 1
         // g0(j) = I_i(2j) I_i(2j+1) mod x^h
-        d.assign((size_t)N, MulDesc{});
-        for (int64_t j = 0; j < N; ++j) {
-            MulDesc &m = d[(size_t)j];
-            m.f = (2 * j) * h; m.g = (2 * j + 1) * h; m.h = j * h;
-            m.lf = m.lg = h; m.lh = h; m.logn = i + 1;
-        }
-        e = launch_list_mul(d.data(), N, inv_at(i), inv_at(i), T1, fam, s);
+        e = launch_mul_strided(i + 1, inv_at(i), 2 * h, h, inv_at(i) + h, 2 * h, h, T1, h, h, N,
+                               fam, s);
         if (e != hipSuccess) break;
         // f(j) = rev(M_(i+1)(j)) mod x^(2h)
-        hipLaunchKernelGGL(k_rev_node, dim3(eblocks(N * 2 * h)), dim3(256), 0, s, node(i + 1),
-                           2 * h, N, T2);
+        hipLaunchKernelGGL(k_rev_node, dim3(eblocks((int64_t)N * 2 * h)), dim3(256), 0, s,
+                           node(i + 1), 2 * h, (int64_t)N, T2);
         // e(j) = f(j) g0(j) mod x^(2h)
-        for (int64_t j = 0; j < N; ++j) {
-            MulDesc &m = d[(size_t)j];
-            m.f = j * 2 * h; m.g = j * h; m.h = j * 2 * h;
-            m.lf = 2 * h; m.lg = h; m.lh = 2 * h; m.logn = i + 2;
-        }
-        e = launch_list_mul(d.data(), N, T2, T1, T3, fam, s);
+        e = launch_mul_strided(i + 2, T2, 2 * h, 2 * h, T1, h, h, T3, 2 * h, 2 * h, N, fam, s);
         if (e != hipSuccess) break;
-        hipLaunchKernelGGL(k_two_minus, dim3(eblocks(N * 2 * h)), dim3(256), 0, s, T3, 2 * h,
-                           N * 2 * h, p);
+        hipLaunchKernelGGL(k_two_minus, dim3(eblocks((int64_t)N * 2 * h)), dim3(256), 0, s, T3,
+                           2 * h, (int64_t)N * 2 * h, p);
         // I_(i+1)(j) = g0(j) (2 - e(j)) mod x^(2h)
-        for (int64_t j = 0; j < N; ++j) {
-            MulDesc &m = d[(size_t)j];
-            m.f = j * h; m.g = j * 2 * h; m.h = j * 2 * h;
-            m.lf = h; m.lg = 2 * h; m.lh = 2 * h; m.logn = i + 2;
-        }
-        e = launch_list_mul(d.data(), N, T1, T3, inv_at(i + 1), fam, s);
+        e = launch_mul_strided(i + 2, T1, h, h, T3, 2 * h, 2 * h, inv_at(i + 1), 2 * h, 2 * h, N,
+                               fam, s);
     }
     if (e != hipSuccess) return e;
     // ---- remainder tree: R_k = F, then levels k-1 .. L0
@@ -186,29 +170,19 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     u32 *A = R0, *Rn = R1;
     for (int i = k - 1; i >= L0 && e == hipSuccess; --i) {
         const int h = 1 << i;
-        const int64_t N = n >> i;                            // nodes of level i
-        hipLaunchKernelGGL(k_div_rev_top, dim3(eblocks(N * h)), dim3(256), 0, s, A, h, N, T1);
+        const int N = (int)(n >> i);                         // nodes of level i
+        hipLaunchKernelGGL(k_div_rev_top, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, A, h,
+                           (int64_t)N, T1);
         // qrev(j) = revtop(j) I_i(j) mod x^h
-        d.assign((size_t)N, MulDesc{});
-        for (int64_t j = 0; j < N; ++j) {
-            MulDesc &m = d[(size_t)j];
-            m.f = j * h; m.g = j * h; m.h = j * h;
-            m.lf = m.lg = h; m.lh = h; m.logn = i + 1;
-        }
-        e = launch_list_mul(d.data(), N, T1, inv_at(i), T2, fam, s);
+        e = launch_mul_strided(i + 1, T1, h, h, inv_at(i), h, h, T2, h, h, N, fam, s);
         if (e != hipSuccess) break;
-        hipLaunchKernelGGL(k_reverse, dim3(eblocks(N * h)), dim3(256), 0, s, T2, h, N, T1);
+        hipLaunchKernelGGL(k_reverse, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, T2, h,
+                           (int64_t)N, T1);
         // qm(j) = q(j) M_i(j) mod x^h (the leading x^h of M vanishes mod x^h)
-        const int64_t moff = tree_level_offset(k, i);
-        for (int64_t j = 0; j < N; ++j) {
-            MulDesc &m = d[(size_t)j];
-            m.f = j * h; m.g = moff + j * (h + 1); m.h = j * h;
-            m.lf = h; m.lg = h; m.lh = h; m.logn = i + 1;
-        }
-        e = launch_list_mul(d.data(), N, T1, tree, T2, fam, s);
+        e = launch_mul_strided(i + 1, T1, h, h, node(i), h + 1, h, T2, h, h, N, fam, s);
         if (e != hipSuccess) break;
-        hipLaunchKernelGGL(k_div_finish, dim3(eblocks(N * h)), dim3(256), 0, s, A, T2, h, N, p,
-                           Rn);
+        hipLaunchKernelGGL(k_div_finish, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, A, T2, h,
+                           (int64_t)N, p, Rn);
         e = hipGetLastError();
         u32 *t = A; A = Rn; Rn = t;
     }

@@ -12,6 +12,8 @@
 //                    the inverse row transform fused; inverse column
 //                    transforms with the truncating store fused;
 //   * k_ntt_lds / k_row_out / k_col_ntt -- natural-order transforms.
+#include <algorithm>
+
 #include "internal.h"
 #include "ntt_core.h"
 
@@ -29,7 +31,8 @@ CM_DEV int xcd_remap(int b, int nb) {
 // whole multiply in one workgroup (C products per workgroup)
 // ---------------------------------------------------------------------------
 // DESC: descriptors from `descs` (or the by-value `one` when descs is null);
-// otherwise row d of F, G, H at offset d * L, full length.  H may alias F.
+// otherwise product d reads F + d * one.f, G + d * one.g (one.lf / one.lg
+// words) and writes H + d * one.h (one.lh words).  H may alias F.
 // Held to 128 VGPRs so that the 2^12- and 2^13-point tiles keep as many
 // workgroups per CU as their LDS allows.
 template <int LOGL, int C, bool DESC>
@@ -58,8 +61,12 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
                 lf[c] = lg[c] = lh[c] = 0;
             }
         } else {
-            off_f[c] = off_g[c] = off_h[c] = (int64_t)d * L;
-            lf[c] = lg[c] = lh[c] = d < ndesc ? L : 0;
+            // product d at d times the strides one.f / one.g / one.h, with
+            // the lengths one.lf / one.lg / one.lh
+            off_f[c] = (int64_t)d * one.f; off_g[c] = (int64_t)d * one.g;
+            off_h[c] = (int64_t)d * one.h;
+            lf[c] = d < ndesc ? one.lf : 0; lg[c] = d < ndesc ? one.lg : 0;
+            lh[c] = d < ndesc ? one.lh : 0;
         }
     }
     // all loads of a thread are issued before the first LDS store
@@ -658,7 +665,7 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     if (!o) return hipErrorOutOfMemory;
     const int64_t n = (int64_t)1 << k;
     ColSet sf{io.f_in, io.f_ws, io.in_stride, io.f_len, n, n};
-    ColSet sg{io.g_in, io.g_ws, io.in_stride, io.g_len, n, n};
+    ColSet sg{io.g_in, io.g_ws, io.g_stride < 0 ? io.in_stride : io.g_stride, io.g_len, n, n};
     hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr,
                            mod_of(fam), s);
     if (e != hipSuccess) return e;
@@ -688,9 +695,54 @@ static hipError_t rows_mul_t(u32 *F, u32 *G, int rows, const Family *fam, hipStr
     using TL = Tile<LOGL, C, false>;
     const int nb = (rows + C - 1) / C;
     ProfScope ps(kProfMulLds, s);
+    const int L = 1 << LOGL;
+    const MulDesc rows_desc{L, L, L, L, L, L, LOGL};
     hipLaunchKernelGGL((k_mul_lds<LOGL, C, false>), dim3(nb), dim3(TL::T), 0, s, F, G, F, nullptr,
-                       rows, MulDesc{}, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
+                       rows, rows_desc, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
     return hipGetLastError();
+}
+
+template <int LOGL>
+static hipError_t mul_strided_t(const u32 *F, const u32 *G, u32 *H, const MulDesc &sd, int count,
+                                const Family *fam, hipStream_t s) {
+    constexpr int C = rows_per_wg<LOGL>();
+    using TL = Tile<LOGL, C, false>;
+    const int nb = (count + C - 1) / C;
+    ProfScope ps(kProfMulLds, s);
+    hipLaunchKernelGGL((k_mul_lds<LOGL, C, false>), dim3(nb), dim3(TL::T), 0, s, F, G, H, nullptr,
+                       count, sd, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
+    return hipGetLastError();
+}
+
+hipError_t launch_mul_strided(int logn, const u32 *F, int64_t fs, int lf, const u32 *G,
+                              int64_t gs, int lg, u32 *H, int64_t hs, int lh, int count,
+                              const Family *fam, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (logn < 4) logn = 4;
+    if (logn <= kMaxLdsLog) {
+        const MulDesc sd{fs, gs, hs, lf, lg, lh, logn};
+        switch (logn) {
+#define CM_CASE(K) case K: return mul_strided_t<K>(F, G, H, sd, count, fam, s);
+            CM_CASE(4) CM_CASE(5) CM_CASE(6) CM_CASE(7) CM_CASE(8) CM_CASE(9) CM_CASE(10)
+            CM_CASE(11) CM_CASE(12) CM_CASE(13)
+#undef CM_CASE
+        }
+        return hipErrorInvalidValue;
+    }
+    // four-step straight from the strided operands into the strided output,
+    // in chunks whose two slabs stay within 1 GiB each
+    const int64_t n = (int64_t)1 << logn;
+    const int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 28) / n);
+    for (int64_t c0 = 0; c0 < count; c0 += chunk) {
+        const int cnt = (int)std::min<int64_t>(chunk, count - c0);
+        u32 *sf = (u32 *)workspace(5, sizeof(u32) * (size_t)(n * cnt), s);
+        u32 *sg = (u32 *)workspace(6, sizeof(u32) * (size_t)(n * cnt), s);
+        if (!sf || !sg) return hipErrorOutOfMemory;
+        FourStepIO io{F + c0 * fs, G + c0 * gs, lf, lg, fs, sf, sg, H + c0 * hs, lh, hs, cnt, gs};
+        hipError_t e = launch_mul_four_step(logn, io, fam, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_rows_mul(int e, u32 *F, u32 *G, int rows, const Family *fam, hipStream_t s) {
