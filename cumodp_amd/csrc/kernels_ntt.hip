@@ -154,7 +154,7 @@ struct ColSet {
 // twiddle w_n^(i2 * k1) when OUTER (the multiply applies it in the row pass
 // instead).  grid.x: column tiles, grid.y: arrays (the first cnt0 from set
 // s0, the rest from s1).
-template <int LOGL1, int C, bool OUTER, bool VEC>
+template <int LOGL1, int C, bool OUTER, bool VEC, bool TOPZERO = false>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
 __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
     ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv,
@@ -170,6 +170,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
     const int64_t n2 = (int64_t)1 << logn2;
     const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
     constexpr int PER = TL::WORDS / TL::T;
+    // TOPZERO: every array is shorter than half the transform, so the rows
+    // i >= 2^(LOGL1-1) are zero; they are neither read nor stored (the first
+    // round treats them as zero)
+    constexpr int HALF_ROWS = TL::L / 2;
     if constexpr (VEC) {
         constexpr int PV = PER / 4;
         uint4 xa[PV];
@@ -177,6 +181,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
         for (int r = 0; r < PV; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
             const int64_t e = i * n2 + c0 + c;
+            if (TOPZERO && i >= HALF_ROWS) continue;
             if (e + 3 < S.in_len) {
                 xa[r] = *(const uint4 *)(X + e);
             } else {
@@ -189,6 +194,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
 #pragma unroll
         for (int r = 0; r < PV; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T);
+            if (TOPZERO && idx / C >= HALF_ROWS) continue;
             lds[TL::phys(idx)] = xa[r].x;
             lds[TL::phys(idx + 1)] = xa[r].y;
             lds[TL::phys(idx + 2)] = xa[r].z;
@@ -200,13 +206,17 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
         for (int r = 0; r < PER; ++r) {
             const int idx = threadIdx.x + r * TL::T, i = idx / C, c = idx % C;
             const int64_t e = i * n2 + c0 + c;
-            xa[r] = e < S.in_len ? X[e] : 0u;
+            xa[r] = (TOPZERO && i >= HALF_ROWS) ? 0u : (e < S.in_len ? X[e] : 0u);
         }
 #pragma unroll
-        for (int r = 0; r < PER; ++r) lds[TL::phys(threadIdx.x + r * TL::T)] = xa[r];
+        for (int r = 0; r < PER; ++r) {
+            if (TOPZERO && (threadIdx.x + r * TL::T) / C >= HALF_ROWS) continue;
+            lds[TL::phys(threadIdx.x + r * TL::T)] = xa[r];
+        }
     }
     __syncthreads();
-    cm_tile_dif<TL>(lds, lv, md);
+    if constexpr (TOPZERO) cm_tile_dif_topzero<TL>(lds, lv, md);
+    else cm_tile_dif<TL>(lds, lv, md);
     if constexpr (VEC) {
 #pragma unroll
         for (int r = 0; r < PER / 4; ++r) {
@@ -568,11 +578,16 @@ static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColFwd, s);
+    const int64_t half = (int64_t)1 << (LOGL1 + logn2 - 1);
+    const bool topzero = LOGL1 >= 2 && s0.in_len <= half && (cnt1 == 0 || s1.in_len <= half);
     if (olo)
         hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true, false>), dim3(nb, cnt0 + cnt1),
                            dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
     else if constexpr (C % 4 == 0) {
-        if (vec_ok(s0) && (cnt1 == 0 || vec_ok(s1)))
+        if (vec_ok(s0) && (cnt1 == 0 || vec_ok(s1)) && topzero)
+            hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, true, true>), dim3(nb, cnt0 + cnt1),
+                               dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+        else if (vec_ok(s0) && (cnt1 == 0 || vec_ok(s1)))
             hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, true>), dim3(nb, cnt0 + cnt1),
                                dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
         else
@@ -669,9 +684,10 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr,
                            mod_of(fam), s);
     if (e != hipSuccess) return e;
-    // row pass: one row of F and G per workgroup (2^k1 * cnt rows)
+    // row pass: one row of F and G per workgroup (2^k1 * cnt rows); the
+    // pointwise scale 2^32 / n is carried by the inverse row twiddles
     const int nrows = (int)(((int64_t)io.cnt << k1));
-    const u32 sc = mul_scale(k, fam->p);
+    const u32 sc = 0u;
     {
         ProfScope ps(kProfRowMul, s);
 #define CM_ROW(K2)                                                                         \

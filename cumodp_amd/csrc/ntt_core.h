@@ -108,8 +108,10 @@ struct Tile {
 };
 
 // R DIF stages (S0 .. S0+R-1 of an L = 2^LOGL transform) on NV register
-// groups that share their positions (and so their twiddles).
-template <int LOGL, int R, int S0, int DLOG, int NV>
+// groups that share their positions (and so their twiddles).  TOPZERO: the
+// upper half of the group (m >= 2^(R-1)) holds zeros, so the first stage is
+// (x, 0) -> (x, x w) and those inputs are never read.
+template <int LOGL, int R, int S0, int DLOG, int NV, bool TOPZERO = false>
 CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lv, Mod md) {
 #pragma unroll
     for (int t = 0; t < R; ++t) {
@@ -119,6 +121,17 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict
         for (int m = 0; m < (1 << R); ++m) {
             if (m & h) continue;
             const int mm = (m & (h - 1)) << DLOG;
+            if (TOPZERO && t == 0) {
+                if (DLOG == 0 && mm == 0) {
+#pragma unroll
+                    for (int a = 0; a < NV; ++a) v[a][m + h] = v[a][m];
+                } else {
+                    const u32 w = lv[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+#pragma unroll
+                    for (int a = 0; a < NV; ++a) v[a][m + h] = cm_mul_tw(v[a][m], w, md.p, md.np);
+                }
+                continue;
+            }
             if (DLOG == 0 && mm == 0) {
 #pragma unroll
                 for (int a = 0; a < NV; ++a) cm_dif_bfly1(v[a][m], v[a][m + h], md);
@@ -188,15 +201,24 @@ struct Round {
 #pragma unroll
             for (int m = 0; m < (1 << R); ++m) lds[a][g.a(m)] = v[a][m];
     }
-    // full forward round on NV tiles, LDS -> LDS
-    template <int NV>
+    // full forward round on NV tiles, LDS -> LDS; TOPZERO (first round
+    // only): the upper half of every vector is zero and is not read
+    template <int NV, bool TOPZERO = false>
     static CM_DEV void dif(u32 *const (&lds)[NV], const u32 *lv, Mod md) {
+        static_assert(!TOPZERO || S0 == 0, "zero upper half only in the first round");
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             G g(threadIdx.x + q * TL::T);
             u32 v[NV][1 << R];
-            load<NV>(lds, g, v);
-            cm_dif_stages<TL::LOGL, R, S0, DLOG, NV>(v, g.gi_lo, lv, md);
+            if (TOPZERO) {
+#pragma unroll
+                for (int a = 0; a < NV; ++a)
+#pragma unroll
+                    for (int m = 0; m < (1 << (R - 1)); ++m) v[a][m] = lds[a][g.a(m)];
+            } else {
+                load<NV>(lds, g, v);
+            }
+            cm_dif_stages<TL::LOGL, R, S0, DLOG, NV, TOPZERO>(v, g.gi_lo, lv, md);
             store<NV>(lds, g, v);
         }
     }
@@ -234,6 +256,16 @@ CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const u32 *lvi, Mod md) {
     }
 }
 
+// Full forward transform of every vector of one tile whose upper halves are
+// zero (and were not stored to LDS).
+template <class TL>
+CM_DEV void cm_tile_dif_topzero(u32 *lds, const u32 *lv, Mod md) {
+    u32 *const t[1] = {lds};
+    Round<TL, 0>::template dif<1, true>(t, lv, md);
+    __syncthreads();
+    cm_dif_rounds<TL, 1, TL::NR, 1>(t, lv, md);
+}
+
 // Full forward transform of every vector of one tile (bit-reversed result).
 template <class TL>
 CM_DEV void cm_tile_dif(u32 *lds, const u32 *lv, Mod md) {
@@ -252,7 +284,9 @@ CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, Mod md) {
 // sc (Montgomery form of 2^32 / N mod p: the two lazy Montgomery products
 // divide by 2^64, sc restores 2^32 and supplies the inverse transform's 1/N)
 // fused with the last forward and first inverse round, then the remaining
-// inverse rounds on A.  Barriers included; the product is in A afterwards.
+// inverse rounds on A.  With sc == 0 the product is left as F G / 2^32 (one
+// Montgomery product) for a caller that applies 2^32 / N later.  Barriers
+// included; the product is in A afterwards.
 template <class TL>
 CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
     u32 *const ab[2] = {A, B};
@@ -268,7 +302,8 @@ CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod 
         u32 w[1][1 << R];
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m)
-            w[0][m] = cm_mul_tw(cm_mont_lazy(v[0][m], v[1][m], md.p, md.np), sc, md.p, md.np);
+            w[0][m] = sc ? cm_mul_tw(cm_mont_lazy(v[0][m], v[1][m], md.p, md.np), sc, md.p, md.np)
+                         : cm_mul_tw(v[0][m], v[1][m], md.p, md.np);
         cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(w, g.gi_lo, lvi, md);
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) A[g.a(m)] = w[0][m];

@@ -150,14 +150,17 @@ const Outer *outer_for(const Family *f, int k) {
     o->hi_f = upload(power_table(cm_host_pow(wn, nlo, p), nhi, p));
     o->lo_i = upload(power_table(wni, nlo, p));
     o->hi_i = upload(power_table(cm_host_pow(wni, nlo, p), nhi, p));
-    // row-pass tables
+    // row-pass tables; the inverse rb table also carries 2^32 / n, the scale
+    // the row pass leaves out of its pointwise product (one Montgomery
+    // product gives F G / 2^32, the inverse transform a factor n)
     const size_t n1 = (size_t)1 << k1, T = ((size_t)1 << k2) / 16;
+    const u32 unscale = cm_host_mul(cm_host_r1(p), cm_host_inv(cm_host_pow(2, k, p), p), p);
     for (int dir = 0; dir < 2; ++dir) {
         const u32 w = dir ? wni : wn;
         std::vector<u32> rb(n1 * T), rc(n1 * 16);
         for (size_t r = 0; r < n1; ++r) {
             const u32 wr = cm_host_pow(w, r, p);                 // w^k1
-            u64 x = cm_host_mont(1, p);
+            u64 x = cm_host_mont(dir ? unscale : 1u, p);
             for (size_t t = 0; t < T; ++t) {
                 rb[r * T + t] = (u32)x;
                 x = x * wr % p;
