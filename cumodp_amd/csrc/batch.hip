@@ -28,12 +28,11 @@ struct TileDesc {
     int32_t pad;
 };
 
-// Mixed-size tile variants (words per tile, threads per workgroup); the
+// Mixed-size tile variants (words per tile, 16 per thread); the
 // tuner picks one per process (cumodp_tune_set_list_tile).  Products of
 // 2^logn <= words points go to the mixed kernel, words / 2^logn per tile.
 constexpr int kMixVariants = 3;
 constexpr int kMixWordsOf[kMixVariants] = {2048, 4096, 8192};
-constexpr int kMixThreadsOf[kMixVariants] = {128, 256, 512};
 int g_mix_variant = 2;   // 8192 words, 512 threads: measured default (4096 uneven pairs)
 
 struct Scales {

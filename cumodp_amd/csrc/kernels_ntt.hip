@@ -161,7 +161,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
     const u32 *__restrict__ olo, const u32 *__restrict__ ohi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
     static_assert(!VEC || (C % 4 == 0 && !OUTER), "vector column access");
-    __shared__ u32 lds[TL::LDS_WORDS];
+    __shared__ __align__(16) u32 lds[TL::LDS_WORDS];
     const int y = blockIdx.y;
     const ColSet &S = y < cnt0 ? s0 : s1;
     const int64_t a = y < cnt0 ? y : y - cnt0;
@@ -195,10 +195,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
         for (int r = 0; r < PV; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T);
             if (TOPZERO && idx / C >= HALF_ROWS) continue;
-            lds[TL::phys(idx)] = xa[r].x;
-            lds[TL::phys(idx + 1)] = xa[r].y;
-            lds[TL::phys(idx + 2)] = xa[r].z;
-            lds[TL::phys(idx + 3)] = xa[r].w;
+            cm_lds_put4<TL>(lds, idx, xa[r]);
         }
     } else {
         u32 xa[PER];
@@ -221,11 +218,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd(
 #pragma unroll
         for (int r = 0; r < PER / 4; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
-            uint4 v;
-            v.x = lds[TL::phys(idx)];
-            v.y = lds[TL::phys(idx + 1)];
-            v.z = lds[TL::phys(idx + 2)];
-            v.w = lds[TL::phys(idx + 3)];
+            const uint4 v = cm_lds_get4<TL>(lds, idx);
             *(uint4 *)(Y + i * n2 + c0 + c) = v;
         }
     } else {
@@ -253,7 +246,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
     ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
     using TL = Tile<LOGL1, C, true>;
     static_assert(!VEC || C % 4 == 0, "vector column access");
-    __shared__ u32 lds[TL::LDS_WORDS];
+    __shared__ __align__(16) u32 lds[TL::LDS_WORDS];
     const int64_t a = blockIdx.y;
     const u32 *X = S.in + a * S.in_stride;
     u32 *Y = S.out + a * S.out_stride;
@@ -271,10 +264,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
 #pragma unroll
         for (int r = 0; r < PV; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T);
-            lds[TL::phys(idx)] = xa[r].x;
-            lds[TL::phys(idx + 1)] = xa[r].y;
-            lds[TL::phys(idx + 2)] = xa[r].z;
-            lds[TL::phys(idx + 3)] = xa[r].w;
+            cm_lds_put4<TL>(lds, idx, xa[r]);
         }
     } else {
         u32 xa[PER];
@@ -293,11 +283,7 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
         for (int r = 0; r < PER / 4; ++r) {
             const int idx = 4 * (threadIdx.x + r * TL::T), i = idx / C, c = idx % C;
             const int64_t e = i * n2 + c0 + c;
-            uint4 v;
-            v.x = lds[TL::phys(idx)];
-            v.y = lds[TL::phys(idx + 1)];
-            v.z = lds[TL::phys(idx + 2)];
-            v.w = lds[TL::phys(idx + 3)];
+            const uint4 v = cm_lds_get4<TL>(lds, idx);
             if (e + 3 < S.out_len) {
                 *(uint4 *)(Y + e) = v;
             } else {

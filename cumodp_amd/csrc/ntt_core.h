@@ -83,12 +83,28 @@ struct Tile {
     // (x ^ 9 ((x >> 6) & 7)): every round's wave access and the linear load
     // and store phases then hit 64 distinct banks (at worst two per bank on a
     // 32-bank LDS), with no padding; the row pass's bank-conflict cycles drop
-    // eightfold.  Other tiles keep one pad word per 16: for the column tiles
-    // the swizzle removes conflicts too, but its address arithmetic does not
-    // fold into LDS instruction offsets and pushes them past 64 VGPRs.
+    // eightfold.
+    // Column tiles of 4 or more columns pad every 64-word block by K = 8 words
+    // (4 for four columns): x + K (x >> 6).  A wave's 64 lanes then hit 64
+    // distinct banks in every round (the last round's lanes stride 64 words,
+    // one block apart, and land K banks apart), a four-word group stays
+    // contiguous and 16-byte aligned for the 128-bit load and store phases,
+    // and each thread's elements of a round sit at constant offsets from its
+    // first (the XOR swizzle removes the conflicts too, but its address
+    // arithmetic does not fold into LDS instruction offsets and pushes the
+    // column kernels past 64 VGPRs).  Other tiles keep one pad word per 16.
     static constexpr bool SWZ = !COLS_ && LOGL_ >= 13;
-    static constexpr int LDS_WORDS = SWZ ? WORDS : WORDS + (WORDS >> 4);
-    static CM_DEV int phys(int x) { return SWZ ? (x ^ (((x >> 6) & 7) * 9)) : cm_pad(x); }
+    static constexpr int BLKPAD = (COLS_ && C_ >= 4) ? (C_ >= 8 ? 8 : 4) : 0;
+    static constexpr int LDS_WORDS =
+        SWZ ? WORDS : (BLKPAD ? WORDS + BLKPAD * (WORDS >> 6) : WORDS + (WORDS >> 4));
+    static CM_DEV int phys(int x) {
+        if constexpr (SWZ) return x ^ (((x >> 6) & 7) * 9);
+        else if constexpr (BLKPAD != 0) return x + BLKPAD * (x >> 6);
+        else return cm_pad(x);
+    }
+    // four consecutive words x .. x+3 (x a multiple of 4) are contiguous and
+    // 16-byte aligned in LDS
+    static constexpr bool QUAD = BLKPAD != 0;
     static constexpr int T = T_;                   // threads per workgroup
     static constexpr int NR = (LOGL_ + 3) / 4;     // rounds
     static constexpr int rsz(int j) { return LOGL_ / NR + (j < LOGL_ % NR ? 1 : 0); }
@@ -106,6 +122,34 @@ struct Tile {
         else { c = g >> (LOGL_ - R); gi = g & ((1 << (LOGL_ - R)) - 1); }
     }
 };
+
+// Words idx .. idx+3 of a tile (idx a multiple of 4): one 128-bit LDS access
+// when the layout keeps them together (the LDS array is 16-byte aligned).
+template <class TL>
+CM_DEV void cm_lds_put4(u32 *lds, int idx, uint4 v) {
+    if constexpr (TL::QUAD) {
+        *(uint4 *)(lds + TL::phys(idx)) = v;
+    } else {
+        lds[TL::phys(idx)] = v.x;
+        lds[TL::phys(idx + 1)] = v.y;
+        lds[TL::phys(idx + 2)] = v.z;
+        lds[TL::phys(idx + 3)] = v.w;
+    }
+}
+
+template <class TL>
+CM_DEV uint4 cm_lds_get4(const u32 *lds, int idx) {
+    if constexpr (TL::QUAD) {
+        return *(const uint4 *)(lds + TL::phys(idx));
+    } else {
+        uint4 v;
+        v.x = lds[TL::phys(idx)];
+        v.y = lds[TL::phys(idx + 1)];
+        v.z = lds[TL::phys(idx + 2)];
+        v.w = lds[TL::phys(idx + 3)];
+        return v;
+    }
+}
 
 // R DIF stages (S0 .. S0+R-1 of an L = 2^LOGL transform) on NV register
 // groups that share their positions (and so their twiddles).  TOPZERO: the
