@@ -125,21 +125,28 @@ def bench_list_multiply(cm, cpu, reps=5):
            "check_graph_eq_batched": bool(np.array_equal(H1, H3)),
            "tuning": {"list_tile_words": L.cumodp_tune_get_list_tile()}}
     if cpu:
-        import oracle
-        oracle.lib()
+        # the reference's own CPU multiply (modpn EX_Mont_Mul, built from its
+        # sources into oracle/_ref) where it was built, else the oracle
+        from oracle import modpn_ref
+        if modpn_ref.available():
+            mul, kind, what = modpn_ref.mul, "reference", "modpn EX_Mont_Mul (plain below " \
+                "degree 138, Montgomery FFT above; modpn/src/FMUL.c:2046-2070)"
+        else:
+            import oracle
+            mul, kind, what = oracle.ntt_polymul, "port", "serial oracle NTT multiply"
         t0, k, coeffs, ok = time.perf_counter(), 0, 0, True
         while k < len(items) and time.perf_counter() - t0 < 8.0:
             it = items[k]
             f = w["F"][it.f_off:it.f_off + it.df + 1]
             g = w["G"][it.g_off:it.g_off + it.dg + 1]
-            h = oracle.ntt_polymul(f, g, p)
+            h = mul(f, g, p)
             ok = ok and bool(np.array_equal(H1[it.h_off:it.h_off + it.df + it.dg + 1], h))
             coeffs += it.df + it.dg + 1
             k += 1
         t = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": coeffs / t, "unit": "coeffs/s", "cores": 1, "kind": "port",
-                               "sample": f"the first {k} pairs, one serial oracle NTT multiply "
-                                         "per pair", "check_bit_exact": ok}
+        out["cpu_baseline"] = {"value": coeffs / t, "unit": "coeffs/s", "cores": 1, "kind": kind,
+                               "sample": f"the first {k} pairs, one {what} per pair",
+                               "check_bit_exact": ok}
     return out
 
 
@@ -503,6 +510,25 @@ def main():
                                "sample": "the full workload: one 2^24-point multiply, "
                                          "median of 2 runs of the oracle's serial NTT multiply",
                                "seconds_per_multiply": tc}
+        from oracle import modpn_ref
+        if modpn_ref.available():
+            # modpn's Montgomery arithmetic is exact below 2^30 only, so the
+            # reference's own CPU FFT multiply is timed at the same 2^24 size
+            # mod 469762049 = 7*2^26+1, on inputs from the same generator
+            log("bench: cpu reference (modpn CPU FFT multiply, 2^24 mod 469762049)")
+            q = 469762049
+            Fq, Gq = F % q, G % q
+            t0 = time.perf_counter()
+            Hq = modpn_ref.fft_mul(Fq, Gq, q)
+            tq = time.perf_counter() - t0
+            Hd = cm.poly_mul(Fq, Gq, q)
+            out["cpu_reference"] = {
+                "value": nout / tq, "unit": "coeffs/s", "cores": 1, "kind": "reference",
+                "sample": "one 2^24-point multiply mod 469762049 by modpn's "
+                          "EX_Mont_FFTMul_OPT2_AS_GENE (modpn/src/FMUL.c:1757), built from the "
+                          "reference's C sources; modpn cannot run mod 2013265921",
+                "seconds_per_multiply": tq,
+                "check_bit_exact_vs_gpu": bool(np.array_equal(Hq, Hd))}
     L.cumodp_stream_destroy(stream)
     dF.free(), dG.free(), dH.free()
     if not args.primary_only:

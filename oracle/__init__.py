@@ -143,8 +143,11 @@ def ntt_polymul(F, G, p):
 
 
 def poly_mul(F, G, p):
-    """Exact product mod p: schoolbook when small, CPU NTT otherwise."""
-    if min(len(F), len(G)) <= 64 or len(F) * len(G) <= 1 << 16:
+    """Exact product mod p: schoolbook when small or when p has no root of
+    unity of the transform's order, CPU NTT otherwise."""
+    e = max(0, (len(F) + len(G) - 2).bit_length())
+    if (min(len(F), len(G)) <= 64 or len(F) * len(G) <= 1 << 16
+            or e > fourier_degree(p)):
         return plain_polymul(F, G, p)
     return ntt_polymul(F, G, p)
 

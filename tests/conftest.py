@@ -28,3 +28,14 @@ def cm():
         build.build()
     cumodp_amd.lib()
     return cumodp_amd
+
+
+@pytest.fixture(scope="session")
+def modpn():
+    """modpn's CPU multiply compiled from the reference's C sources
+    (oracle/_ref); skipped where the reference tree was absent at build time."""
+    from oracle import modpn_ref
+    if not modpn_ref.available():
+        pytest.skip("oracle/_ref/libmodpn_fmul.so not built (reference tree absent)")
+    modpn_ref.lib()
+    return modpn_ref
