@@ -150,6 +150,61 @@ def bench_list_multiply(cm, cpu, reps=5):
     return out
 
 
+def bench_plain_crossover(cm, cpu, reps=5):
+    """Schoolbook (cumodp_list_plainmul_dev) against NTT (cumodp_list_fftmul_dev)
+    list products of equal length N, 4096 pairs per size, mod 469762049: where
+    the plain multiply stops winning.  CPU leg: modpn's plain multiply (its
+    EX_Mont_Mul below degree 138) where oracle/_ref was built, else the
+    oracle's schoolbook, on a sample of the N = 64 pairs."""
+    p = 469762049
+    pairs = 4096
+    rng = cm.SplitMix64(77)
+    out = {"workload": "4096 pairs of equal length N mod 469762049, plain vs NTT list multiply",
+           "sizes": {}}
+    L = cm.lib()
+    for N in (16, 32, 64, 128, 256, 512):
+        F = cm.uniform_mod(rng, pairs * N, p)
+        G = cm.uniform_mod(rng, pairs * N, p)
+        items, nf, ng, nh = cm.make_items(np.full(pairs, N - 1), np.full(pairs, N - 1))
+        dF, dG = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(G)
+        dH1, dH2 = cm.DeviceArray(nh), cm.DeviceArray(nh)
+        run_p = lambda: cm.list_plainmul_dev(items, dF, dG, dH1, p)
+        run_n = lambda: cm.list_fftmul_dev(items, dF, dG, dH2, p)
+        run_p(), run_n()
+        tp = timed(run_p, reps, cm.synchronize)
+        tn = timed(run_n, reps, cm.synchronize)
+        H1, H2 = dH1.download(), dH2.download()
+        for d in (dF, dG, dH1, dH2):
+            d.free()
+        out["sizes"][str(N)] = {"plain_ms": tp * 1e3, "ntt_ms": tn * 1e3,
+                                "plain_over_ntt_speed": tn / tp,
+                                "check_plain_eq_ntt": bool(np.array_equal(H1, H2))}
+        if N == 64:
+            keep = (F, G, H1, items)
+    won = [int(n) for n, v in out["sizes"].items() if v["plain_over_ntt_speed"] > 1.0]
+    out["plain_faster_up_to_N"] = max(won) if won else None
+    if cpu:
+        from oracle import modpn_ref
+        F, G, H1, items = keep
+        if modpn_ref.available():
+            mul, kind, what = modpn_ref.mul, "reference", "modpn EX_Mont_Mul (plain below degree 138)"
+        else:
+            import oracle
+            mul, kind, what = oracle.plain_polymul, "port", "the oracle's schoolbook"
+        t0, k, coeffs, ok = time.perf_counter(), 0, 0, True
+        while k < len(items) and time.perf_counter() - t0 < 4.0:
+            it = items[k]
+            h = mul(F[it.f_off:it.f_off + 64], G[it.g_off:it.g_off + 64], p)
+            ok = ok and bool(np.array_equal(H1[it.h_off:it.h_off + 127], h))
+            coeffs += 127
+            k += 1
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": coeffs / t, "unit": "coeffs/s", "cores": 1, "kind": kind,
+                               "sample": f"{k} of the N = 64 pairs, {what} each",
+                               "check_bit_exact": ok}
+    return out
+
+
 def bench_tree(cm, cpu, reps=5):
     """Subproduct tree over 2^16 points mod 469762049, all 17 levels.  CPU
     leg: the oracle's levels 1..8 (schoolbook per node), compared with the
@@ -534,7 +589,9 @@ def main():
     if not args.primary_only:
         sec = {}
         cpu = not args.no_cpu_baseline
-        for key, fn in (("list_multiply", bench_list_multiply), ("subproduct_tree", bench_tree),
+        for key, fn in (("list_multiply", bench_list_multiply),
+                        ("plain_crossover", bench_plain_crossover),
+                        ("subproduct_tree", bench_tree),
                         ("fast_evaluation", bench_eval),
                         ("two_convolution", bench_two_convolution)):
             log(f"bench: {key}")

@@ -60,6 +60,7 @@ CXX_SYMBOLS = {
     "bivariate_stockham_host": "_Z23bivariate_stockham_hostPiiiiii",
     "inverse_bivariate_stockham_host": "_Z31inverse_bivariate_stockham_hostPiiiiii",
     "bi_stockham_poly_mul_dev": "_Z24bi_stockham_poly_mul_deviiiiPiS_i",
+    "mulCUDA": "_Z7mulCUDAPiS_S_iiii",
 }
 
 
@@ -123,6 +124,10 @@ def lib():
         L.cumodp_fftmul_uni_dev.restype = err
         L.cumodp_list_fftmul_dev.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp, s, vp]
         L.cumodp_list_fftmul_dev.restype = err
+        L.cumodp_list_plainmul_dev.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp, s, vp]
+        L.cumodp_list_plainmul_dev.restype = err
+        L.cumodp_plain_mul_dev.argtypes = [vp, vp, s, vp, s, s, vp]
+        L.cumodp_plain_mul_dev.restype = err
         L.cumodp_list_fftmul_dev_percall.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp,
                                                      s, vp]
         L.cumodp_list_fftmul_dev_percall.restype = err
@@ -323,6 +328,34 @@ def list_fftmul_dev(items, Fd, Gd, Hd, p, stream=None, per_call=False):
     fn = "cumodp_list_fftmul_dev_percall" if per_call else "cumodp_list_fftmul_dev"
     _check(fn, getattr(lib(), fn)(items, len(items), Fd.addr(), Gd.addr(), Hd.addr(), p,
                                   C.c_void_p(stream)))
+
+
+def list_plainmul_dev(items, Fd, Gd, Hd, p, stream=None):
+    """cumodp_list_plainmul_dev: schoolbook products of the same items."""
+    _check("cumodp_list_plainmul_dev", lib().cumodp_list_plainmul_dev(
+        items, len(items), Fd.addr(), Gd.addr(), Hd.addr(), p, C.c_void_p(stream)))
+
+
+def plain_mul_dev(Cd, Ad, n, Bd, m, p, stream=None):
+    """cumodp_plain_mul_dev on device addresses (DeviceArray or int)."""
+    def a(x):
+        v = x.addr() if hasattr(x, "addr") else x
+        return v if isinstance(v, C.c_void_p) else C.c_void_p(v)
+    _check("cumodp_plain_mul_dev", lib().cumodp_plain_mul_dev(
+        a(Cd), a(Ad), n, a(Bd), m, p, C.c_void_p(stream)))
+
+
+def mulCUDA(A, B, p, C_out=None):
+    """The reference's plain multiply mulCUDA(A, B, C, n, m, Prime, verify)
+    (src/opt_plain_mul.cu:258): returns C = A * B mod p (len(A)+len(B)-1
+    words).  C_out, when given, is the int32 output array (left untouched on
+    failure)."""
+    A, B = _i32(A), _i32(B)
+    out = np.zeros(len(A) + len(B) - 1, np.int32) if C_out is None else C_out
+    fn = cxx("mulCUDA")
+    fn.restype = None
+    fn(_ptr(A), _ptr(B), _ptr(out), len(A), len(B), p, 1)
+    return out
 
 
 def make_items(df, dg):

@@ -13,6 +13,7 @@
 #include "../../include/fft_aux.h"
 #include "../../include/list_inv_stockham.h"
 #include "../../include/list_stockham.h"
+#include "../../include/opt_plain_mul.h"
 #include "../../include/stockham.h"
 #include "internal.h"
 
@@ -100,6 +101,70 @@ cumodp_err mul_host_impl(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg,
     ce = mul_dev_impl(dH, nout, dF, df + 1, dG, dg + 1, (u32)p, st);
     if (ce != CUMODP_SUCCESS) return ce;
     he = hipMemcpyAsync(H, dH, sizeof(u32) * (size_t)nout, hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    return CUMODP_SUCCESS;
+}
+
+// ---- schoolbook multiply (any odd modulus) --------------------------------
+// Products up to 2^30 output coefficients; operands must be canonical.
+constexpr int64_t kPlainMaxOut = (int64_t)1 << 30;
+
+cumodp_err check_plain_modulus(u32 p) {
+    if (p < 3 || (p & 1) == 0 || p >= (1u << 31))
+        return fail(CUMODP_ASSUMPTION_ERROR, "modulus must be odd, in [3, 2^31)");
+    return CUMODP_SUCCESS;
+}
+
+// descs[i] for F[f .. f+lf) x G[g .. g+lg) -> H[h .. h+lf+lg-1)
+cumodp_err plain_dev_impl(std::vector<MulDesc> &descs, const u32 *F, const u32 *G, u32 *H, u32 p,
+                          hipStream_t s) {
+    cumodp_err ce = check_plain_modulus(p);
+    if (ce != CUMODP_SUCCESS) return ce;
+    for (MulDesc &d : descs) {
+        if (d.lf < 1 || d.lg < 1) return fail(CUMODP_ASSUMPTION_ERROR, "empty operand");
+        if ((int64_t)d.lf + d.lg - 1 > kPlainMaxOut)
+            return fail(CUMODP_ASSUMPTION_ERROR, "plain product longer than 2^30");
+        d.lh = d.lf + d.lg - 1;
+        d.logn = d.lg > d.lf ? 1 : 0;   // sweep the shorter operand
+    }
+    hipError_t he = launch_plain_list(descs.data(), (int64_t)descs.size(), F, G, H, p, s);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "plain multiply", he);
+    return CUMODP_SUCCESS;
+}
+
+// Shorter operands than this many coefficients go to the schoolbook kernel
+// in mulCUDA; longer ones to the NTT multiply when p has the roots for it.
+constexpr int64_t kPlainCrossover = 128;
+
+// mulCUDA semantics (src/opt_plain_mul.cu:258-384): C[0 .. n+m-2] = A * B
+// mod p for host arrays of n and m coefficients.
+cumodp_err plain_host_impl(const sfixn *A, sfixn n, const sfixn *B, sfixn m, sfixn *C, sfixn p) {
+    if (n < 1 || m < 1) return fail(CUMODP_ASSUMPTION_ERROR, "empty operand");
+    cumodp_err ce = check_plain_modulus((u32)p);
+    if (ce != CUMODP_SUCCESS) return ce;
+    const int64_t nc = (int64_t)n + m - 1;
+    if (nc > kPlainMaxOut) return fail(CUMODP_ASSUMPTION_ERROR, "product longer than 2^30");
+    const int e = ceil_log2(nc);
+    const bool ntt = std::min(n, m) >= kPlainCrossover && e <= kMaxLog &&
+                     e <= cm_fourier_degree((u32)p);
+    std::lock_guard<std::mutex> lk(api_mutex());
+    u32 *buf = (u32 *)workspace(2, sizeof(u32) * (size_t)(n + m + nc), 0);
+    if (!buf) return fail(CUMODP_OUT_OF_MEMORY, "staging buffer");
+    u32 *dA = buf, *dB = buf + n, *dC = dB + m;
+    hipStream_t st = 0;
+    hipError_t he = hipMemcpyAsync(dA, A, sizeof(u32) * (size_t)n, hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dB, B, sizeof(u32) * (size_t)m, hipMemcpyHostToDevice, st);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    if (ntt) {
+        ce = mul_dev_impl(dC, nc, dA, n, dB, m, (u32)p, st);
+    } else {
+        std::vector<MulDesc> d(1);
+        d[0].f = 0; d[0].g = 0; d[0].h = 0; d[0].lf = n; d[0].lg = m;
+        ce = plain_dev_impl(d, dA, dB, dC, (u32)p, st);
+    }
+    if (ce != CUMODP_SUCCESS) return ce;
+    he = hipMemcpyAsync(C, dC, sizeof(u32) * (size_t)nc, hipMemcpyDeviceToHost, st);
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
     return CUMODP_SUCCESS;
@@ -714,6 +779,31 @@ cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, c
     return CUMODP_SUCCESS;
 }
 
+cumodp_err cumodp_list_plainmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
+                                    const sfixn *G, sfixn *H, sfixn p, void *stream) {
+    if (count < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative count");
+    if (count == 0) return CUMODP_SUCCESS;
+    if (count > INT32_MAX) return fail(CUMODP_ASSUMPTION_ERROR, "too many items");
+    std::vector<MulDesc> d((size_t)count);
+    for (int64_t i = 0; i < count; ++i) {
+        const cumodp_mul_item &it = items[i];
+        if (it.df < 0 || it.dg < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative degree");
+        d[(size_t)i].f = it.f_off; d[(size_t)i].g = it.g_off; d[(size_t)i].h = it.h_off;
+        d[(size_t)i].lf = it.df + 1; d[(size_t)i].lg = it.dg + 1;
+    }
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return plain_dev_impl(d, (const u32 *)F, (const u32 *)G, (u32 *)H, (u32)p, as_stream(stream));
+}
+
+cumodp_err cumodp_plain_mul_dev(sfixn *C, const sfixn *A, sfixn n, const sfixn *B, sfixn m,
+                                sfixn p, void *stream) {
+    if (n < 1 || m < 1) return fail(CUMODP_ASSUMPTION_ERROR, "empty operand");
+    std::vector<MulDesc> d(1);
+    d[0].f = 0; d[0].g = 0; d[0].h = 0; d[0].lf = n; d[0].lg = m;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return plain_dev_impl(d, (const u32 *)A, (const u32 *)B, (u32 *)C, (u32)p, as_stream(stream));
+}
+
 // One cumodp_fftmul_uni_dev-equivalent call per item (the plain per-call form
 // a caller would write as a loop), for comparison with the batched engine.
 cumodp_err cumodp_list_fftmul_dev_percall(const cumodp_mul_item *items, int64_t count,
@@ -791,6 +881,11 @@ void stockham_poly_mul_dev(sfixn n, sfixn e, sfixn *F, sfixn *G, sfixn p) {
 void stockham_poly_mul_host(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg,
                             const sfixn *G, sfixn p) {
     report("stockham_poly_mul_host", mul_host_impl(dh, H, df, F, dg, G, p));
+}
+
+void mulCUDA(sfixn *A, sfixn *B, sfixn *C, sfixn n, sfixn m, sfixn Prime, sfixn verify) {
+    (void)verify;   // the reference prints sizes and time unless verify == 1
+    report("mulCUDA", plain_host_impl(A, n, B, m, C, Prime));
 }
 
 void list_stockham_host(sfixn *X, sfixn m, sfixn k, sfixn w, sfixn p) {

@@ -56,6 +56,19 @@ typedef struct {
 
 CUMODP_API cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, const sfixn *F,
                                   const sfixn *G, sfixn *H, sfixn p, void *stream);
+/* Schoolbook products of the same items (no transform, no root of unity):
+ * any odd modulus 3 <= p < 2^31, canonical operands, at most 2^30 output
+ * words per item.  One wavefront computes 256 consecutive outputs of one
+ * item.  Replaces the reference's plain multiply, mulCUDA
+ * (src/opt_plain_mul.cu:258-384), and its tree-level list form
+ * listPlainMulGpu (src/fastPolyEvaluation.cu:198-280), batched over items of
+ * any lengths. */
+CUMODP_API cumodp_err cumodp_list_plainmul_dev(const cumodp_mul_item *items, int64_t count,
+                                               const sfixn *F, const sfixn *G, sfixn *H,
+                                               sfixn p, void *stream);
+/* C[0 .. n+m-2] = A[0 .. n-1] * B[0 .. m-1] mod p on device buffers. */
+CUMODP_API cumodp_err cumodp_plain_mul_dev(sfixn *C, const sfixn *A, sfixn n, const sfixn *B,
+                                           sfixn m, sfixn p, void *stream);
 /* The same products as one cumodp_fftmul_uni_dev call per item (the plain
  * per-call form), kept for comparisons with the batched engine. */
 CUMODP_API cumodp_err cumodp_list_fftmul_dev_percall(const cumodp_mul_item *items, int64_t count,
