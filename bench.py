@@ -123,7 +123,8 @@ def bench_list_multiply(cm, cpu, reps=5):
            "batched_over_per_call": tc / tb,
            "check_batched_eq_per_call": bool(np.array_equal(H1, H2)),
            "check_graph_eq_batched": bool(np.array_equal(H1, H3)),
-           "tuning": {"list_tile_words": L.cumodp_tune_get_list_tile()}}
+           "tuning": {"list_tile_words": L.cumodp_tune_get_list_tile(),
+                      "list_plain_cutoff": L.cumodp_tune_get_list_plain()}}
     if cpu:
         # the reference's own CPU multiply (modpn EX_Mont_Mul, built from its
         # sources into oracle/_ref) where it was built, else the oracle

@@ -149,6 +149,9 @@ def lib():
         L.cumodp_tune_set_list_tile.argtypes = [s]
         L.cumodp_tune_set_list_tile.restype = err
         L.cumodp_tune_get_list_tile.restype = s
+        L.cumodp_tune_set_list_plain.argtypes = [s]
+        L.cumodp_tune_set_list_plain.restype = err
+        L.cumodp_tune_get_list_plain.restype = s
         L.cumodp_tune_set_col_tile.argtypes = [s]
         L.cumodp_tune_set_col_tile.restype = err
         L.cumodp_tune_get_col_tile.restype = s

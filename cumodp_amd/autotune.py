@@ -1,6 +1,6 @@
 """Measured choice of the library's run-time tunables on the current GPU.
 
-There are three tunables:
+There are four tunables:
 
   * The four-step split of a 2^k-point transform (k > 13): rows of 2^k2
     words, with k2 in {12, 13, 14}, and columns of 2^(k-k2) words.
@@ -12,6 +12,9 @@ There are three tunables:
     threads, or 8192 words with 512 threads. The smaller tile gives more,
     smaller workgroups per CU and less lockstep between them, but each tile
     row is a shorter segment in memory. `cumodp_tune_set_col_tile`.
+  * The plain cut-off of the list multiply: products whose shorter operand
+    has at most this many coefficients run on the schoolbook kernel, the
+    rest on the NTT. `cumodp_tune_set_list_plain`.
 
 Each candidate is timed on the operation it affects. The fastest candidate
 is applied to the library, and the choices can be saved to and loaded from
@@ -103,6 +106,28 @@ def tune_list_tile(reps=5, log=None, pairs=4096):
     return b
 
 
+def tune_list_plain(reps=5, log=None, pairs=4096, cutoffs=(0, 64, 128, 192, 256, 384)):
+    """Time the uneven list multiply (the 4096-pair workload) for each plain
+    cut-off; apply and return the best."""
+    L = lib()
+    w = workload4(pairs=pairs)
+    items, nf, ng, nh = make_items(w["df"], w["dg"])
+    dF, dG, dH = DeviceArray.from_host(w["F"]), DeviceArray.from_host(w["G"]), DeviceArray(nh)
+    fn = lambda: list_fftmul_dev(items, dF, dG, dH, w["p"])
+    times = {}
+    for c in cutoffs:
+        _check("cumodp_tune_set_list_plain", L.cumodp_tune_set_list_plain(c))
+        times[c] = _median_time(fn, reps)
+    b = min(times, key=times.get)
+    _check("cumodp_tune_set_list_plain", L.cumodp_tune_set_list_plain(b))
+    if log:
+        log("autotune: list multiply plain cut-off: " +
+            ", ".join(f"{c}: {t * 1e3:.3f} ms" for c, t in times.items()) + f" -> {b}")
+    for d in (dF, dG, dH):
+        d.free()
+    return b
+
+
 def tune_col_tile(k=24, reps=5, p=P2, log=None):
     """Time the 2^k multiply with each column tile size; apply and return the
     best."""
@@ -130,14 +155,16 @@ def tune_col_tile(k=24, reps=5, p=P2, log=None):
 def tune_all(log=None, ks=range(14, 25)):
     col = tune_col_tile(log=log)
     return {"col_tile_words": col, "split": tune_split(ks=ks, log=log),
-            "list_tile_words": tune_list_tile(log=log)}
+            "list_tile_words": tune_list_tile(log=log),
+            "list_plain_cutoff": tune_list_plain(log=log)}
 
 
 def current():
     L = lib()
     return {"split": {k: L.cumodp_tune_get_split(k) for k in range(14, 27)},
             "list_tile_words": L.cumodp_tune_get_list_tile(),
-            "col_tile_words": L.cumodp_tune_get_col_tile()}
+            "col_tile_words": L.cumodp_tune_get_col_tile(),
+            "list_plain_cutoff": L.cumodp_tune_get_list_plain()}
 
 
 def apply(choices):
@@ -150,13 +177,17 @@ def apply(choices):
     if "col_tile_words" in choices:
         _check("cumodp_tune_set_col_tile",
                L.cumodp_tune_set_col_tile(int(choices["col_tile_words"])))
+    if "list_plain_cutoff" in choices:
+        _check("cumodp_tune_set_list_plain",
+               L.cumodp_tune_set_list_plain(int(choices["list_plain_cutoff"])))
 
 
 def save(choices, path):
     with open(path, "w") as f:
         json.dump({"split": {str(k): v for k, v in choices["split"].items()},
                    "list_tile_words": choices["list_tile_words"],
-                   "col_tile_words": choices.get("col_tile_words", 16384)}, f, indent=1)
+                   "col_tile_words": choices.get("col_tile_words", 16384),
+                   "list_plain_cutoff": choices.get("list_plain_cutoff", 256)}, f, indent=1)
 
 
 def load(path):

@@ -739,6 +739,14 @@ cumodp_err cumodp_tune_set_list_tile(int words) {
 
 int cumodp_tune_get_list_tile(void) { return list_tile_words(); }
 
+cumodp_err cumodp_tune_set_list_plain(int cutoff) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return set_list_plain_cutoff(cutoff) ? CUMODP_SUCCESS
+                                         : fail(CUMODP_ASSUMPTION_ERROR, "plain cutoff: 0 .. 1024");
+}
+
+int cumodp_tune_get_list_plain(void) { return list_plain_cutoff(); }
+
 cumodp_err cumodp_tune_set_col_tile(int words) {
     std::lock_guard<std::mutex> lk(api_mutex());
     return set_col_tile_words(words) ? CUMODP_SUCCESS

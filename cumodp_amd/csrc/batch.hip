@@ -145,8 +145,47 @@ bool set_list_tile_words(int words) {
 
 int list_tile_words() { return kMixWordsOf[g_mix_variant]; }
 
+// Products whose shorter operand has at most this many coefficients run on
+// the schoolbook kernel (plain.hip) in list multiplies: there the la * lb
+// products cost less than three transforms of 2^ceil(log2(la + lb - 1))
+// points.  0 sends everything to the NTT.  The tuner sets it.
+int g_plain_cutoff = 256;
+
+bool set_list_plain_cutoff(int c) {
+    if (c < 0 || c > 1024) return false;
+    g_plain_cutoff = c;
+    return true;
+}
+
+int list_plain_cutoff() { return g_plain_cutoff; }
+
+static hipError_t launch_list_mul_ntt(const MulDesc *descs, int64_t count, const u32 *F,
+                                      const u32 *G, u32 *H, const Family *fam, hipStream_t s);
+
 hipError_t launch_list_mul(const MulDesc *descs, int64_t count, const u32 *F, const u32 *G,
                            u32 *H, const Family *fam, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (g_plain_cutoff <= 0) return launch_list_mul_ntt(descs, count, F, G, H, fam, s);
+    std::vector<MulDesc> plain, rest;
+    for (int64_t i = 0; i < count; ++i) {
+        const MulDesc &d = descs[i];
+        if (std::min(d.lf, d.lg) <= g_plain_cutoff) {
+            MulDesc q = d;
+            q.logn = d.lg > d.lf ? 1 : 0;   // sweep the shorter operand
+            plain.push_back(q);
+        } else {
+            rest.push_back(d);
+        }
+    }
+    if (!plain.empty()) {
+        hipError_t e = launch_plain_list(plain.data(), (int64_t)plain.size(), F, G, H, fam->p, s);
+        if (e != hipSuccess) return e;
+    }
+    return launch_list_mul_ntt(rest.data(), (int64_t)rest.size(), F, G, H, fam, s);
+}
+
+static hipError_t launch_list_mul_ntt(const MulDesc *descs, int64_t count, const u32 *F,
+                                      const u32 *G, u32 *H, const Family *fam, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     // order: small sizes (tiled), then 2^13, then four-step classes by size
     std::vector<int64_t> order((size_t)count);

@@ -11,9 +11,11 @@
 // Level i comes from level i-1 by the same monic identity the reference uses
 // (:601-650): with A = x^h + a and B = x^h + b (h = 2^(i-1), deg a, b < h),
 //     A B = x^(2h) + x^h (a + b) + a b,
-// where all products a b of a level are one call of the batched multiply
-// (mixed-size LDS kernel, or four-step for 2^i > 2^13); small levels use a
-// direct schoolbook kernel.
+// where all products a b of a level are one call of a batched multiply:
+// levels up to 5 use a one-thread-per-coefficient schoolbook kernel, levels
+// whose halves have up to 256 coefficients the wavefront schoolbook kernel
+// (plain.hip), larger ones the NTT list multiply (mixed-size LDS kernel, or
+// four-step for 2^i > 2^13).
 #include <vector>
 
 #include "internal.h"
@@ -79,6 +81,10 @@ static int tblocks(int64_t n) {
     return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
+// Levels whose halves have at most this many non-leading coefficients use
+// the schoolbook kernel (plain.hip).
+constexpr int kTreePlainMaxH = 256;
+
 hipError_t launch_subproduct_tree(const u32 *pts, int k, u32 p, u32 *tree, const Family *fam,
                                   hipStream_t s) {
     const int64_t n = (int64_t)1 << k;
@@ -95,6 +101,9 @@ hipError_t launch_subproduct_tree(const u32 *pts, int k, u32 p, u32 *tree, const
             e = hipGetLastError();
             continue;
         }
+        // a b of up to 256 coefficients each: schoolbook products (faster than
+        // the transform there); larger: the batched NTT multiply
+        const bool plain = h <= kTreePlainMaxH;
         d.resize((size_t)nprod);
         for (int64_t j = 0; j < nprod; ++j) {
             MulDesc &m = d[(size_t)j];
@@ -103,9 +112,10 @@ hipError_t launch_subproduct_tree(const u32 *pts, int k, u32 p, u32 *tree, const
             m.h = oc + j * (2 * h + 1);
             m.lf = m.lg = h;             // the non-leading parts a and b
             m.lh = 2 * h - 1;
-            m.logn = i;                  // 2h points hold the degree 2h-2 product
+            m.logn = plain ? 0 : i;      // NTT: 2h points hold the degree 2h-2 product
         }
-        e = launch_list_mul(d.data(), nprod, tree, tree, tree, fam, s);
+        e = plain ? launch_plain_list(d.data(), nprod, tree, tree, tree, p, s)
+                  : launch_list_mul(d.data(), nprod, tree, tree, tree, fam, s);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_tree_combine, dim3(tblocks(nprod * (h + 1))), dim3(256), 0, s, tree,
                            op, oc, h, nprod, p);

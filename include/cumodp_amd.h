@@ -180,10 +180,16 @@ CUMODP_API cumodp_err cumodp_tune_set_split(int k, int k2);
 /* the current split k2 of 2^k-point transforms (0 when k needs none) */
 CUMODP_API int cumodp_tune_get_split(int k);
 /* words per workgroup tile of the list multiply's mixed-size kernel: 2048
- * (128 threads), 4096 (256, the default) or 8192 (512); products of up to that
- * many points share tiles, larger ones run one per workgroup or four-step */
+ * (128 threads), 4096 (256) or 8192 (512, the default); products of up to
+ * that many points share tiles, larger ones run one per workgroup or
+ * four-step */
 CUMODP_API cumodp_err cumodp_tune_set_list_tile(int words);
 CUMODP_API int cumodp_tune_get_list_tile(void);
+/* cumodp_list_fftmul_dev sends products whose shorter operand has at most
+ * `cutoff` coefficients to the schoolbook kernel (same results; default
+ * 256; 0 sends every product to the NTT; at most 1024) */
+CUMODP_API cumodp_err cumodp_tune_set_list_plain(int cutoff);
+CUMODP_API int cumodp_tune_get_list_plain(void);
 /* words per workgroup tile of the four-step multiply's column passes: 16384
  * (the default; 1024 threads) or 8192 (512 threads, more workgroups) */
 CUMODP_API cumodp_err cumodp_tune_set_col_tile(int words);

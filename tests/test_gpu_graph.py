@@ -102,6 +102,14 @@ def test_autotune_choices_keep_results(cm, orc):
         assert np.array_equal(outs[0][it.h_off:it.h_off + it.df + it.dg + 1],
                               orc.ntt_polymul(f, g, w["p"]))
         assert L.cumodp_tune_set_list_tile(1000) == cm.CUMODP_ASSUMPTION_ERROR
+        # every plain cut-off (schoolbook for short operands) gives the same products
+        for cut in (0, 16, 300, 1024):
+            assert L.cumodp_tune_set_list_plain(cut) == 0
+            dH = cm.DeviceArray(nh)
+            cm.list_fftmul_dev(items, dF, dG, dH, w["p"])
+            cm.synchronize()
+            assert np.array_equal(outs[0], dH.download()), cut
+        assert L.cumodp_tune_set_list_plain(2000) == cm.CUMODP_ASSUMPTION_ERROR
         # both column tiles give the same 2^20 product
         x = cm.workload2(logn=20)
         dX, dY = cm.DeviceArray.from_host(x["F"]), cm.DeviceArray.from_host(x["G"])
