@@ -2,7 +2,9 @@
 // C++-linkage drop-ins (include/stockham.h, list_stockham.h, fft_aux.h,
 // list_inv_stockham.h) over the gfx950 kernels.
 #include <algorithm>
+#include <chrono>
 #include <climits>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -262,6 +264,23 @@ void report(const char *fn, cumodp_err e) {
     if (e != CUMODP_SUCCESS) fprintf(stderr, "%s: %s\n", fn, g_last_error.c_str());
 }
 
+// The reference prints timings to stdout on every call
+// (src/stockham.cu:600, src/two_convolution_poly_mul.cu:30,50,
+// src/opt_plain_mul.cu:367-369); here they appear only with CUMODP_VERBOSE
+// set to a non-zero value.
+bool verbose() {
+    static const bool v = [] {
+        const char *e = getenv("CUMODP_VERBOSE");
+        return e && *e && *e != '0';
+    }();
+    return v;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -480,10 +499,12 @@ cumodp_err cumodp_two_convolution_poly_mul(const uint64_t *a, int64_t na, int wa
     hipError_t he = hipMemcpy(da, a, sizeof(u64) * (size_t)(na * wa), hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemcpy(db, b, sizeof(u64) * (size_t)(nb * wb), hipMemcpyHostToDevice);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    const double t0 = verbose() ? now_s() : 0.0;
     cumodp_err ce = tc_dev_impl(da, na, wa, db, nb, wb, dc, wc, std::max(ba, bb), nullptr);
     if (ce != CUMODP_SUCCESS) return ce;
     he = hipMemcpy(c, dc, sizeof(u64) * (size_t)(nc * wc), hipMemcpyDeviceToHost);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy out", he);
+    if (verbose()) printf("\ttwo_convolution_poly_mul took %.3f ms\n", (now_s() - t0) * 1e3);
     return CUMODP_SUCCESS;
 }
 
@@ -888,12 +909,19 @@ void stockham_poly_mul_dev(sfixn n, sfixn e, sfixn *F, sfixn *G, sfixn p) {
 
 void stockham_poly_mul_host(sfixn dh, sfixn *H, sfixn df, const sfixn *F, sfixn dg,
                             const sfixn *G, sfixn p) {
-    report("stockham_poly_mul_host", mul_host_impl(dh, H, df, F, dg, G, p));
+    const double t0 = verbose() ? now_s() : 0.0;
+    const cumodp_err e = mul_host_impl(dh, H, df, F, dg, G, p);
+    report("stockham_poly_mul_host", e);
+    if (verbose() && e == CUMODP_SUCCESS) printf("FFT-based: %g\n", now_s() - t0);
 }
 
 void mulCUDA(sfixn *A, sfixn *B, sfixn *C, sfixn n, sfixn m, sfixn Prime, sfixn verify) {
-    (void)verify;   // the reference prints sizes and time unless verify == 1
-    report("mulCUDA", plain_host_impl(A, n, B, m, C, Prime));
+    const double t0 = verbose() ? now_s() : 0.0;
+    const cumodp_err e = plain_host_impl(A, n, B, m, C, Prime);
+    report("mulCUDA", e);
+    // the reference prints "n m seconds" unless verify == 1
+    if (verbose() && verify != 1 && e == CUMODP_SUCCESS)
+        printf("%d %d %g\n", (int)n, (int)m, now_s() - t0);
 }
 
 void list_stockham_host(sfixn *X, sfixn m, sfixn k, sfixn w, sfixn p) {

@@ -103,13 +103,16 @@ __global__ __launch_bounds__(T) void k_mul_mixed(
 
 // zero-padded copy of operand `which` (0: F, 1: G) of descriptor row y into
 // row y of a slab of n-word rows
-__global__ void k_gather(u32 *__restrict__ slab, int64_t n, const u32 *__restrict__ src,
-                         const MulDesc *__restrict__ d, int which) {
+// Row y of the slab (stride n words) gets the first m words of operand y,
+// zero-padded: m is the longest operand of the batch, and the column pass
+// reads no further.
+__global__ void k_gather(u32 *__restrict__ slab, int64_t n, int64_t m,
+                         const u32 *__restrict__ src, const MulDesc *__restrict__ d, int which) {
     const MulDesc dd = d[blockIdx.y];
     const int64_t off = which ? dd.g : dd.f;
     const int len = which ? dd.lg : dd.lf;
     u32 *row = slab + (int64_t)blockIdx.y * n;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
          i += (int64_t)gridDim.x * blockDim.x)
         row[i] = i < len ? src[off + i] : 0u;
 }
@@ -273,16 +276,25 @@ static hipError_t launch_list_mul_ntt(const MulDesc *descs, int64_t count, const
             u32 *sf = (u32 *)workspace(5, sizeof(u32) * (size_t)(n * cnt), s);
             u32 *sg = (u32 *)workspace(6, sizeof(u32) * (size_t)(n * cnt), s);
             if (!sf || !sg) return hipErrorOutOfMemory;
+            // the slabs hold only the longest operand of each side (the
+            // column pass zero-pads the rest, and skips the zero upper half
+            // when both fit in n/2); the inverse pass stores the longest output
+            int64_t mf = 1, mg = 1, mh = 1;
+            for (int64_t q = c0; q < c0 + cnt; ++q) {
+                mf = std::max<int64_t>(mf, sorted[(size_t)q].lf);
+                mg = std::max<int64_t>(mg, sorted[(size_t)q].lg);
+                mh = std::max<int64_t>(mh, sorted[(size_t)q].lh);
+            }
             const unsigned gx = (unsigned)std::min<int64_t>(n / 1024, 64);
             {
             ProfScope ps(kProfGatherScatter, s);
-            hipLaunchKernelGGL(k_gather, dim3(gx, cnt), dim3(1024), 0, s, sf, n, F,
+            hipLaunchKernelGGL(k_gather, dim3(gx, cnt), dim3(1024), 0, s, sf, n, mf, F,
                                d_descs + c0, 0);
-            hipLaunchKernelGGL(k_gather, dim3(gx, cnt), dim3(1024), 0, s, sg, n, G,
+            hipLaunchKernelGGL(k_gather, dim3(gx, cnt), dim3(1024), 0, s, sg, n, mg, G,
                                d_descs + c0, 1);
             }
             e = hipGetLastError();
-            FourStepIO io{sf, sg, n, n, n, sf, sg, sf, n, n, cnt};
+            FourStepIO io{sf, sg, mf, mg, n, sf, sg, sf, mh, n, cnt};
             if (e == hipSuccess) e = launch_mul_four_step(lg, io, fam, s);
             if (e != hipSuccess) return e;
             {

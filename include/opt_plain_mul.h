@@ -14,8 +14,8 @@
  * reference requires n >= m; either order works here.  Products whose shorter
  * operand has at least 128 coefficients run through the NTT multiply when
  * Prime has the roots for it (same result); the rest run the schoolbook
- * kernel.  verify is accepted and ignored (the reference prints n, m and the
- * time unless verify == 1). */
+ * kernel.  With CUMODP_VERBOSE set and verify != 1, prints "n m seconds" to
+ * stdout as the reference does unconditionally (src/opt_plain_mul.cu:367-369). */
 CUMODP_API void mulCUDA(sfixn *A, sfixn *B, sfixn *C, sfixn n, sfixn m, sfixn Prime,
                         sfixn verify);
 

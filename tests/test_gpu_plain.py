@@ -151,3 +151,31 @@ def test_plain_rejects_bad_arguments(cm):
     items = (cm.MulItem * 1)()
     items[0].df = -1
     assert L.cumodp_list_plainmul_dev(items, 1, None, None, None, 7, None) != cm.CUMODP_SUCCESS
+
+
+def test_timing_prints_are_opt_in(cm):
+    """CUMODP_VERBOSE=1 turns on the reference's stdout timing lines
+    (src/stockham.cu:600, src/opt_plain_mul.cu:367-369); by default the
+    library prints nothing.  One child process per setting."""
+    import os
+    import subprocess
+    import sys
+    code = ("import numpy as np, cumodp_amd as cm\n"
+            "F = np.arange(1, 301, dtype=np.int32); G = np.arange(2, 202, dtype=np.int32)\n"
+            "H = np.zeros(500, np.int32)\n"
+            "cm.stockham_poly_mul_host(499, H, 299, F, 199, G, 469762049)\n"
+            "cm.mulCUDA(F[:10], G[:5], 469762049)\n"
+            "C = np.zeros(14, np.int32); fn = cm.cxx('mulCUDA'); fn.restype = None\n"
+            "fn(cm._ptr(F), cm._ptr(G), cm._ptr(C), 10, 5, 469762049, 0)\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = {}
+    for v in ("0", "1"):
+        env = dict(os.environ, CUMODP_VERBOSE=v)
+        r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs[v] = r.stdout
+    assert outs["0"] == ""
+    lines = outs["1"].splitlines()
+    assert lines[0].startswith("FFT-based: ")
+    assert len(lines) == 2 and lines[1].split()[:2] == ["10", "5"]   # verify=1 call is silent
