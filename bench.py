@@ -522,13 +522,28 @@ def main():
     achieved = algo / avg_s / 1e9
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
     # (calibrated on a copy kernel), kept in profiles/
-    traffic = None
+    # and the VALU instructions per launch (SQ_INSTS_VALU): the issue rate,
+    # not HBM, bounds these kernels
+    traffic, counted = None, {}
+    valu_peak = 1024 * 2.2e9 / 4          # wave-instructions/s: 4 cycles each per SIMD
     tpath = os.path.join(ROOT, "profiles", "traffic_2e24.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f)["kernels"].get(name, {}).get("traffic_bytes")
-    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1]}
-               for k, v in prof.items()}
+            tj = json.load(f)
+        counted = tj["kernels"]
+        valu_peak = tj.get("valu_peak_per_s", valu_peak)
+        traffic = counted.get(name, {}).get("traffic_bytes")
+    kernels = {}
+    for k, v in prof.items():
+        a = v[0] / v[1] * 1e-3
+        kernels[k] = {"avg_us": round(a * 1e6, 2), "launches": v[1]}
+        vi = counted.get(k, {}).get("valu_insts")
+        if vi:
+            kernels[k]["valu_issue_frac"] = round(vi / a / valu_peak, 3)
+    vi = counted.get(name, {}).get("valu_insts")
+    valu = None if not vi else {
+        "insts_per_launch": vi, "achieved": vi / avg_s / 1e9, "peak": valu_peak / 1e9,
+        "unit": "G wave-instructions/s", "frac": vi / avg_s / valu_peak}
     whole = 48 * n / (ms_per_step * 1e-3) / 1e9
 
     out = {
@@ -550,7 +565,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "algo_bytes_per_launch": algo,
-                     "avg_launch_us": avg_s * 1e6},
+                     "avg_launch_us": avg_s * 1e6, "valu_issue": valu},
         "whole_multiply_hbm_frac_48n": whole / HBM_PEAK_GBS,
         "kernels": kernels,
         "tuning": {"autotuned": bool(args.autotune), "split_k2": L.cumodp_tune_get_split(24),
