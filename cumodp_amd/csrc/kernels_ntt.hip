@@ -301,6 +301,58 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
     }
 }
 
+// The same forward and inverse column passes of the multiply with the first
+// and last rounds reading and writing global memory directly (no LDS staging
+// pass, two barriers fewer; any alignment).  Tiles of two or more rounds.
+template <int LOGL1, int C, bool TOPZERO>
+__global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
+__attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd_d(
+    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv, Mod md) {
+    using TL = Tile<LOGL1, C, true>;
+    static_assert(TL::NR >= 2, "direct column pass needs two rounds");
+    __shared__ __align__(16) u32 lds[TL::LDS_WORDS];
+    const int y = blockIdx.y;
+    const ColSet &S = y < cnt0 ? s0 : s1;
+    const int64_t a = y < cnt0 ? y : y - cnt0;
+    const int64_t n2 = (int64_t)1 << logn2;
+    const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
+    const u32 *X = S.in + a * S.in_stride + c0;
+    u32 *Y = S.out + a * S.out_stride + c0;
+    const int64_t lim = S.in_len - c0;
+    constexpr int64_t rows = TOPZERO ? TL::L / 2 : TL::L;
+    if (lim >= (rows - 1) * n2 + C)      // every element round 0 reads is in range
+        cm_col_dif_first<TL, TOPZERO, false>(lds, X, n2, lim, lv, md);
+    else
+        cm_col_dif_first<TL, TOPZERO, true>(lds, X, n2, lim, lv, md);
+    __syncthreads();
+    u32 *const t[1] = {lds};
+    cm_dif_rounds<TL, 1, TL::NR - 1, 1>(t, lv, md);
+    cm_col_dif_last<TL>(lds, Y, n2, lv, md);
+}
+
+template <int LOGL1, int C>
+__global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
+__attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv_d(
+    ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
+    using TL = Tile<LOGL1, C, true>;
+    static_assert(TL::NR >= 2, "direct column pass needs two rounds");
+    __shared__ __align__(16) u32 lds[TL::LDS_WORDS];
+    const int64_t a = blockIdx.y;
+    const int64_t n2 = (int64_t)1 << logn2;
+    const int c0 = xcd_remap(blockIdx.x, gridDim.x) * C;
+    const u32 *X = S.in + a * S.in_stride + c0;
+    u32 *Y = S.out + a * S.out_stride + c0;
+    const int64_t lim = S.out_len - c0;
+    cm_col_dit_first<TL>(lds, X, n2, lvi, md);
+    __syncthreads();
+    u32 *const t[1] = {lds};
+    cm_dit_rounds<TL, TL::NR - 2, 1, 1>(t, lvi, md);
+    if (lim >= (int64_t)(TL::L - 1) * n2 + C)
+        cm_col_dit_last<TL, false>(lds, Y, n2, lim, lvi, md);
+    else
+        cm_col_dit_last<TL, true>(lds, Y, n2, lim, lvi, md);
+}
+
 // Row pass of the four-step multiply: row r of F and G (r = a * n1 + j, the
 // row holding frequency k1 = brev(j)); elements i = t + T m are multiplied by
 // the outer twiddle w_n^(k1 i) = rb[k1 T + t] * rc[16 k1 + m] on the way in,
@@ -333,19 +385,18 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        const int i = t + T * m;
         const u32 c = rc[k1 * 16 + m];
-        A[TL::phys(i)] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md.p, md.np);
-        B[TL::phys(i)] = cm_mul_tw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md.p, md.np);
+        fa[m] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md.p, md.np);
+        ga[m] = cm_mul_tw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md.p, md.np);
     }
-    __syncthreads();
-    cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
+    // elements t + T m are thread t's group of the first forward and the last
+    // inverse round: those rounds run on the registers
+    cm_row_multiply<TL>(A, B, fa, ga, lv, lvi, md, sc);
     const u32 bi = rbi[(int64_t)k1 * T + t];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        const int i = t + T * m;
         const u32 c = rci[k1 * 16 + m];
-        Fr[i] = cm_mul_tw(cm_mont_lazy(A[TL::phys(i)], c, md.p, md.np), bi, md.p, md.np);
+        Fr[t + T * m] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), bi, md.p, md.np);
     }
 }
 
@@ -556,6 +607,13 @@ bool set_col_tile_words(int words) {
 
 int col_tile_words() { return g_col_words; }
 
+// Column passes of the multiply with direct global first/last rounds
+// (k_col_fwd_d, k_col_inv_d) or with LDS staging (k_col_fwd, k_col_inv).
+static bool g_col_direct = [] {
+    const char *e = getenv("CUMODP_COL_DIRECT");
+    return !(e && *e == '0');
+}();
+
 template <int LOGL1, int WORDS>
 static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
                             const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
@@ -566,10 +624,23 @@ static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     ProfScope ps(kProfColFwd, s);
     const int64_t half = (int64_t)1 << (LOGL1 + logn2 - 1);
     const bool topzero = LOGL1 >= 2 && s0.in_len <= half && (cnt1 == 0 || s1.in_len <= half);
-    if (olo)
+    if (olo) {
         hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true, false>), dim3(nb, cnt0 + cnt1),
                            dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
-    else if constexpr (C % 4 == 0) {
+        return hipGetLastError();
+    }
+    if constexpr (TL::NR >= 2) {
+        if (g_col_direct) {
+            if (topzero)
+                hipLaunchKernelGGL((k_col_fwd_d<LOGL1, C, true>), dim3(nb, cnt0 + cnt1),
+                                   dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, md);
+            else
+                hipLaunchKernelGGL((k_col_fwd_d<LOGL1, C, false>), dim3(nb, cnt0 + cnt1),
+                                   dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, md);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (C % 4 == 0) {
         if (vec_ok(s0) && (cnt1 == 0 || vec_ok(s1)) && topzero)
             hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, true, true>), dim3(nb, cnt0 + cnt1),
                                dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
@@ -604,6 +675,13 @@ static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *f
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColInv, s);
+    if constexpr (TL::NR >= 2) {
+        if (g_col_direct) {
+            hipLaunchKernelGGL((k_col_inv_d<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
+                               logn2, fam->lvi, mod_of(fam));
+            return hipGetLastError();
+        }
+    }
     if constexpr (C % 4 == 0) {
         if (vec_ok(S)) {
             hipLaunchKernelGGL((k_col_inv<LOGL1, C, true>), dim3(nb, cnt), dim3(TL::T), 0, s, S,

@@ -323,6 +323,98 @@ CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, Mod md) {
     cm_dit_rounds<TL, TL::NR - 1, 0, 1>(t, lvi, md);
 }
 
+// Column tiles (COLS) whose first and last rounds talk to global memory
+// directly: element (row, c) of the tile is X[row n2 + c], X at the tile's
+// first column.  Each thread's round-group is one column, so a wave reads or
+// writes C-column row segments; the row-dependent part of an address is
+// uniform (m 2^DLOG n2), which keeps the per-lane offset in one 32-bit
+// register.  No LDS staging pass and no barrier precede the first round or
+// follow the last.
+
+// Forward round 0 from global memory into LDS.  CHECK: elements at or beyond
+// lim (relative to X) read as zero.  TOPZERO: rows >= L/2 are zero and not
+// read.
+template <class TL, bool TOPZERO, bool CHECK>
+CM_DEV void cm_col_dif_first(u32 *lds, const u32 *X, int64_t n2, int64_t lim, const u32 *lv,
+                             Mod md) {
+    using RD = Round<TL, 0>;
+    constexpr int R = RD::R, M = TOPZERO ? (1 << (R - 1)) : (1 << R);
+#pragma unroll
+    for (int q = 0; q < RD::GPT; ++q) {
+        typename RD::G g(threadIdx.x + q * TL::T);
+        const u32 off = (u32)g.i0 * (u32)n2 + (u32)g.c;
+        u32 v[1][1 << R];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
+            const u32 *Xm = X + rowoff;
+            v[0][m] = (!CHECK || rowoff + off < lim) ? Xm[off] : 0u;
+        }
+        cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1, TOPZERO>(v, g.gi_lo, lv, md);
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) lds[g.a(m)] = v[0][m];
+    }
+}
+
+// Last forward round from LDS straight to global memory (all rows).
+template <class TL>
+CM_DEV void cm_col_dif_last(const u32 *lds, u32 *Y, int64_t n2, const u32 *lv, Mod md) {
+    using RD = Round<TL, TL::NR - 1>;
+    constexpr int R = RD::R;
+#pragma unroll
+    for (int q = 0; q < RD::GPT; ++q) {
+        typename RD::G g(threadIdx.x + q * TL::T);
+        const u32 off = (u32)g.i0 * (u32)n2 + (u32)g.c;
+        u32 v[1][1 << R];
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) v[0][m] = lds[g.a(m)];
+        cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(v, g.gi_lo, lv, md);
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) (Y + (int64_t)(m << RD::DLOG) * n2)[off] = v[0][m];
+    }
+}
+
+// First inverse round (round NR-1) from global memory into LDS (all rows).
+template <class TL>
+CM_DEV void cm_col_dit_first(u32 *lds, const u32 *X, int64_t n2, const u32 *lvi, Mod md) {
+    using RD = Round<TL, TL::NR - 1>;
+    constexpr int R = RD::R;
+#pragma unroll
+    for (int q = 0; q < RD::GPT; ++q) {
+        typename RD::G g(threadIdx.x + q * TL::T);
+        const u32 off = (u32)g.i0 * (u32)n2 + (u32)g.c;
+        u32 v[1][1 << R];
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) v[0][m] = (X + (int64_t)(m << RD::DLOG) * n2)[off];
+        cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(v, g.gi_lo, lvi, md);
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) lds[g.a(m)] = v[0][m];
+    }
+}
+
+// Last inverse round (round 0) from LDS straight to global memory.  CHECK:
+// elements at or beyond lim (relative to Y) are not stored.
+template <class TL, bool CHECK>
+CM_DEV void cm_col_dit_last(const u32 *lds, u32 *Y, int64_t n2, int64_t lim, const u32 *lvi,
+                            Mod md) {
+    using RD = Round<TL, 0>;
+    constexpr int R = RD::R;
+#pragma unroll
+    for (int q = 0; q < RD::GPT; ++q) {
+        typename RD::G g(threadIdx.x + q * TL::T);
+        const u32 off = (u32)g.i0 * (u32)n2 + (u32)g.c;
+        u32 v[1][1 << R];
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) v[0][m] = lds[g.a(m)];
+        cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(v, g.gi_lo, lvi, md);
+#pragma unroll
+        for (int m = 0; m < (1 << R); ++m) {
+            const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
+            if (!CHECK || rowoff + off < lim) (Y + rowoff)[off] = v[0][m];
+        }
+    }
+}
+
 // Multiply of two tiles A (first operand, becomes the product) and B:
 // forward transforms of both (shared twiddles), pointwise product scaled by
 // sc (Montgomery form of 2^32 / N mod p: the two lazy Montgomery products
@@ -331,10 +423,13 @@ CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, Mod md) {
 // inverse rounds on A.  With sc == 0 the product is left as F G / 2^32 (one
 // Montgomery product) for a caller that applies 2^32 / N later.  Barriers
 // included; the product is in A afterwards.
+// The middle of a tile multiply: the last forward round of A and B, the
+// pointwise product (scaled by sc as in cm_tile_multiply) and the first
+// inverse round, LDS to LDS into A, then a barrier.
 template <class TL>
-CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
+CM_DEV void cm_tile_pointwise_round(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md,
+                                    u32 sc) {
     u32 *const ab[2] = {A, B};
-    cm_dif_rounds<TL, 0, TL::NR - 1, 2>(ab, lv, md);
     using RD = Round<TL, TL::NR - 1>;
     constexpr int R = RD::R;
 #pragma unroll
@@ -353,6 +448,51 @@ CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod 
         for (int m = 0; m < (1 << R); ++m) A[g.a(m)] = w[0][m];
     }
     __syncthreads();
+}
+
+template <class TL>
+CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
+    u32 *const ab[2] = {A, B};
+    cm_dif_rounds<TL, 0, TL::NR - 1, 2>(ab, lv, md);
+    cm_tile_pointwise_round<TL>(A, B, lv, lvi, md, sc);
     u32 *const a1[1] = {A};
     cm_dit_rounds<TL, TL::NR - 2, 0, 1>(a1, lvi, md);
+}
+
+// The same multiply for one row tile (C == 1, 16 elements per thread) whose
+// round-0 group, elements t + T m of thread t, is in registers on entry (fa
+// for A, ga for B) and whose product's round-0 group is left in fa: the
+// first forward and the last inverse round skip the LDS staging pass and its
+// barrier.  sc as in cm_tile_multiply.
+template <class TL>
+CM_DEV void cm_row_multiply(u32 *A, u32 *B, u32 (&fa)[16], const u32 (&ga)[16], const u32 *lv,
+                            const u32 *lvi, Mod md, u32 sc) {
+    using R0 = Round<TL, 0>;
+    static_assert(TL::C == 1 && R0::R == 4 && R0::GPT == 1 && TL::NR >= 2,
+                  "row tiles of 16 elements per thread");
+    const typename R0::G g(threadIdx.x);
+    u32 v[2][16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        v[0][m] = fa[m];
+        v[1][m] = ga[m];
+    }
+    cm_dif_stages<TL::LOGL, 4, 0, R0::DLOG, 2>(v, g.gi_lo, lv, md);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        A[g.a(m)] = v[0][m];
+        B[g.a(m)] = v[1][m];
+    }
+    __syncthreads();
+    u32 *const ab[2] = {A, B};
+    cm_dif_rounds<TL, 1, TL::NR - 1, 2>(ab, lv, md);
+    cm_tile_pointwise_round<TL>(A, B, lv, lvi, md, sc);
+    u32 *const a1[1] = {A};
+    cm_dit_rounds<TL, TL::NR - 2, 1, 1>(a1, lvi, md);
+    u32 w[1][16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) w[0][m] = A[g.a(m)];
+    cm_dit_stages<TL::LOGL, 4, 0, R0::DLOG, 1>(w, g.gi_lo, lvi, md);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) fa[m] = w[0][m];
 }
