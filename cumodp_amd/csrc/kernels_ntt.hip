@@ -78,6 +78,19 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
         fa[r] = i < lf[c] ? F[off_f[c] + i] : 0u;
         ga[r] = i < lg[c] ? G[off_g[c] + i] : 0u;
     }
+    if constexpr (C == 1 && Round<TL, 0>::R == 4 && Round<TL, 0>::GPT == 1 && TL::NR >= 2) {
+        // one product per workgroup: the loaded words t + T r are thread t's
+        // group of the first forward and the last inverse round, which run
+        // on the registers
+        cm_row_multiply<TL>(A, B, fa, ga, lv, lvi, md, sc);
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int i = threadIdx.x + r * TL::T;
+            if (i < lh[0]) H[off_h[0] + i] = fa[r];
+        }
+        for (int i = L + threadIdx.x; i < lh[0]; i += TL::T) H[off_h[0] + i] = 0u;
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
