@@ -1,0 +1,44 @@
+// accum.h -- sums of products mod p in 64-bit accumulators.
+//
+// A sum of products of canonical residues (p < 2^31) is kept in a u64 and
+// folded, acc -> hi * c + lo with c = 2^32 mod p, which is congruent mod p and
+// at most B1 = (2^32 - 1)(c + 1).  Four products fit between folds for every
+// p < 2^31: with k = floor(2^32 / p) >= 2 and c = 2^32 - k p,
+// B1 + 4 (p-1)^2 = 2^64 - k 2^32 p + 4 p^2 + O(p) < 2^64, since 4 p < k 2^32
+// (k = 2: p < 2^31; k >= 3: p <= 2^32 / k < k 2^30).  A value is reduced to
+// [0, p) once at the end: two folds, one Montgomery reduction, one Montgomery
+// product by 2^64 mod p.  Any odd modulus works (no root of unity needed).
+#pragma once
+
+#include "modarith.h"
+
+namespace cm {
+
+struct PlainMod {
+    u32 p, np, c32, r2;   // modulus, -p^-1 mod 2^32, 2^32 mod p, 2^64 mod p
+};
+
+__device__ __forceinline__ u64 plain_fold(u64 acc, u32 c32) {
+    return (u64)(u32)(acc >> 32) * c32 + (u32)acc;
+}
+
+// acc (any 64-bit value) -> acc mod p
+__device__ __forceinline__ u32 plain_reduce(u64 acc, const PlainMod &m) {
+    acc = plain_fold(plain_fold(acc, m.c32), m.c32);    // < 2^62 + 2^33
+    const u32 q = (u32)acc * m.np;
+    const u32 r = (u32)((acc + (u64)q * m.p) >> 32);    // acc / 2^32 mod p, < 2^32
+    return cm_mul_tw(r, m.r2, m.p, m.np);               // r 2^64 / 2^32 = acc mod p
+}
+
+
+// host: the constants for modulus p (odd, 3 <= p < 2^31)
+static inline PlainMod plain_mod(u32 p) {
+    PlainMod m;
+    m.p = p;
+    m.np = cm_host_npinv(p);
+    m.c32 = (u32)(((u64)1 << 32) % p);
+    m.r2 = (u32)((unsigned __int128)((u64)m.c32 * m.c32) % p);
+    return m;
+}
+
+}  // namespace cm

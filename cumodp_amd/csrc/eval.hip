@@ -20,6 +20,7 @@
 // Every batched product of a level is one launch_mul_strided call: the
 // level's products share one shape, so they need no descriptors (one LDS
 // multiply per workgroup up to 2^13 points, batched four-step above).
+#include "accum.h"
 #include "internal.h"
 
 namespace cm {
@@ -31,20 +32,34 @@ static int eblocks(int64_t n) {
     return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
-// I_L0(j) = rev(M)^-1 mod x^h for the nodes of level L0 (h = 2^L0): with
-// f = rev(M) (f_0 = 1, f_s = M_(h-s)), g_0 = 1 and g_t = -sum_(s=1..t) f_s g_(t-s).
-__global__ void k_inv_base(const u32 *__restrict__ M, int h, int64_t nodes, u32 p,
+// I_L0(j) = rev(M)^-1 mod x^H for the nodes of level L0 (H = 2^L0): with
+// f = rev(M) (f_0 = 1, f_s = M_(H-s)), g_0 = 1 and g_t = -sum_(s=1..t) f_s g_(t-s).
+// One thread per node, node and inverse in registers; the sums run in 64-bit
+// accumulators folded every four products (accum.h).
+template <int H>
+__global__ void k_inv_base(const u32 *__restrict__ M, int64_t nodes, PlainMod pm,
                            u32 *__restrict__ inv) {
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nodes;
          j += (int64_t)gridDim.x * blockDim.x) {
-        const u32 *m = M + j * (h + 1);
-        u32 *g = inv + j * h;
+        const u32 *m = M + j * (H + 1);
+        u32 f[H], g[H];
+#pragma unroll
+        for (int s = 1; s < H; ++s) f[s] = m[H - s];
         g[0] = 1u;
-        for (int t = 1; t < h; ++t) {
+#pragma unroll
+        for (int t = 1; t < H; ++t) {
             u64 acc = 0;
-            for (int s = 1; s <= t; ++s) acc = (acc + (u64)m[h - s] * g[t - s]) % p;
-            g[t] = acc ? p - (u32)acc : 0u;
+#pragma unroll
+            for (int s = 1; s <= t; ++s) {
+                acc += (u64)f[s] * g[t - s];
+                if ((s & 3) == 0) acc = plain_fold(acc, pm.c32);
+            }
+            const u32 r = plain_reduce(acc, pm);
+            g[t] = r ? pm.p - r : 0u;
         }
+        u32 *o = inv + j * H;
+#pragma unroll
+        for (int t = 0; t < H; ++t) o[t] = g[t];
     }
 }
 
@@ -54,8 +69,8 @@ __global__ void k_rev_node(const u32 *__restrict__ M, int h2, int64_t nodes,
     const int64_t total = nodes * h2;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = g / h2;
-        const int t = (int)(g % h2);
+        const int64_t j = g >> __builtin_ctz((unsigned)h2);   // h2 is a power of two
+        const int t = (int)(g & (h2 - 1));
         f[g] = M[j * (h2 + 1) + (h2 - t)];
     }
 }
@@ -65,7 +80,7 @@ __global__ void k_two_minus(u32 *__restrict__ e, int h2, int64_t total, u32 p) {
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
          g += (int64_t)gridDim.x * blockDim.x) {
         const u32 v = e[g];
-        if (g % h2 == 0) e[g] = cm_sub(2u % p, v, p);
+        if ((g & (h2 - 1)) == 0) e[g] = cm_sub(2u % p, v, p);
         else e[g] = v ? p - v : 0u;
     }
 }
@@ -76,8 +91,8 @@ __global__ void k_div_rev_top(const u32 *__restrict__ A, int h, int64_t nodes,
     const int64_t total = nodes * h;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = g / h;
-        const int t = (int)(g % h);
+        const int64_t j = g >> __builtin_ctz((unsigned)h);    // h is a power of two
+        const int t = (int)(g & (h - 1));
         out[g] = A[(j >> 1) * (2 * h) + (2 * h - 1 - t)];
     }
 }
@@ -88,8 +103,8 @@ __global__ void k_reverse(const u32 *__restrict__ in, int h, int64_t nodes,
     const int64_t total = nodes * h;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = g / h;
-        const int t = (int)(g % h);
+        const int64_t j = g >> __builtin_ctz((unsigned)h);
+        const int t = (int)(g & (h - 1));
         out[g] = in[j * h + (h - 1 - t)];
     }
 }
@@ -100,22 +115,25 @@ __global__ void k_div_finish(const u32 *__restrict__ A, const u32 *__restrict__ 
     const int64_t total = nodes * h;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = g / h;
-        const int t = (int)(g % h);
+        const int64_t j = g >> __builtin_ctz((unsigned)h);
+        const int t = (int)(g & (h - 1));
         R[g] = cm_sub(A[(j >> 1) * (2 * h) + t], qm[g], p);
     }
 }
 
-// values_x = R_L0(x / h)(point x) by Horner's rule
+// values_x = R_L0(x / h)(point x) by Horner's rule, one Montgomery product
+// by z 2^32 per coefficient
 __global__ void k_horner(const u32 *__restrict__ R, int h, const u32 *__restrict__ pts,
-                         int64_t n, u32 p, u32 *__restrict__ val) {
+                         int64_t n, PlainMod pm, u32 *__restrict__ val) {
+    const int lh = __builtin_ctz((unsigned)h);   // h is a power of two
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n;
          x += (int64_t)gridDim.x * blockDim.x) {
-        const u32 *r = R + (x / h) * h;
-        const u64 z = pts[x] % p;
-        u64 acc = 0;
-        for (int t = h - 1; t >= 0; --t) acc = (acc * z + r[t]) % p;
-        val[x] = (u32)acc;
+        const u32 *r = R + ((x >> lh) << lh);
+        const u32 zr = cm_mul_tw(pts[x] % pm.p, pm.r2, pm.p, pm.np);   // z 2^32 mod p
+        u32 acc = 0;
+        for (int t = h - 1; t >= 0; --t)
+            acc = cm_add(cm_mul_tw(acc, zr, pm.p, pm.np), r[t], pm.p);
+        val[x] = acc;
     }
 }
 
@@ -141,8 +159,22 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     // ---- subinverse tree, levels L0 .. k-1
     {
         const int h = 1 << L0;
-        hipLaunchKernelGGL(k_inv_base, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0), h,
-                           n >> L0, p, inv_at(L0));
+        static_assert(kEvalBase == 4, "base inverse kernel is instantiated for 16 words");
+        if (h == 16)
+            hipLaunchKernelGGL(k_inv_base<16>, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0),
+                               n >> L0, plain_mod(p), inv_at(L0));
+        else if (h == 8)
+            hipLaunchKernelGGL(k_inv_base<8>, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0),
+                               n >> L0, plain_mod(p), inv_at(L0));
+        else if (h == 4)
+            hipLaunchKernelGGL(k_inv_base<4>, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0),
+                               n >> L0, plain_mod(p), inv_at(L0));
+        else if (h == 2)
+            hipLaunchKernelGGL(k_inv_base<2>, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0),
+                               n >> L0, plain_mod(p), inv_at(L0));
+        else
+            hipLaunchKernelGGL(k_inv_base<1>, dim3(eblocks(n >> L0)), dim3(256), 0, s, node(L0),
+                               n >> L0, plain_mod(p), inv_at(L0));
         e = hipGetLastError();
     }
     for (int i = L0; i + 1 <= k - 1 && e == hipSuccess; ++i) {
@@ -187,8 +219,8 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         u32 *t = A; A = Rn; Rn = t;
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_horner, dim3(eblocks(n)), dim3(256), 0, s, A, 1 << L0, pts, n, p,
-                       values);
+    hipLaunchKernelGGL(k_horner, dim3(eblocks(n)), dim3(256), 0, s, A, 1 << L0, pts, n,
+                       plain_mod(p), values);
     return hipGetLastError();
 }
 

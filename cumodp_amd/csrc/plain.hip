@@ -14,17 +14,12 @@
 // is swept four coefficients at a time; each step reads the four B values as
 // one broadcast 128-bit LDS load and the four A values that enter the lane's
 // window as one aligned 128-bit LDS load (lanes 16 bytes apart: no bank
-// conflicts), then issues 16 v_mad_u64_u32.  Accumulators stay below 2^64 by
-// folding acc -> hi * c + lo, c = 2^32 mod p (congruent mod p), after every
-// four products.  A folded value is at most B1 = (2^32 - 1)(c + 1), and
-// B1 + 4 (p-1)^2 < 2^64 for every p < 2^31: with k = floor(2^32 / p) >= 2 and
-// c = 2^32 - k p the sum is 2^64 - k 2^32 p + 4 p^2 + O(p), below 2^64 when
-// 4 p < k 2^32 (k = 2: p < 2^31; k >= 3: p <= 2^32 / k < k 2^30).  Each output
-// is reduced to [0, p) once at the end (two folds, one Montgomery reduction,
-// one Montgomery product by 2^64 mod p).
+// conflicts), then issues 16 v_mad_u64_u32 into 64-bit accumulators that
+// are folded after every four products (accum.h).
 #include <cstring>
 #include <vector>
 
+#include "accum.h"
 #include "internal.h"
 
 namespace cm {
@@ -40,37 +35,36 @@ struct PlainChunk {
     int32_t c0;     // first output coefficient
 };
 
-struct PlainMod {
-    u32 p, np, c32, r2;   // modulus, -p^-1 mod 2^32, 2^32 mod p, 2^64 mod p
-};
-
-__device__ __forceinline__ u64 plain_fold(u64 acc, u32 c32) {
-    return (u64)(u32)(acc >> 32) * c32 + (u32)acc;
-}
-
-// acc (any 64-bit value) -> acc mod p
-__device__ __forceinline__ u32 plain_reduce(u64 acc, const PlainMod &m) {
-    acc = plain_fold(plain_fold(acc, m.c32), m.c32);    // < 2^62 + 2^33
-    const u32 q = (u32)acc * m.np;
-    const u32 r = (u32)((acc + (u64)q * m.p) >> 32);    // acc / 2^32 mod p, < 2^32
-    return cm_mul_tw(r, m.r2, m.p, m.np);               // r 2^64 / 2^32 = acc mod p
-}
-
 // Descriptor d: operands F + d.f (d.lf words) and G + d.g (d.lg words),
 // output H + d.h (d.lh = lf + lg - 1 words).  The sweep runs over the shorter
 // operand: d.logn != 0 marks a descriptor whose G operand is the longer one.
 // Operands are canonical residues.  One workgroup (one wavefront) per chunk
-// of 256 outputs.
+// of 256 outputs.  UNIFORM: every item has the shape of `one`, whose f, g, h
+// fields are the strides between items (item = block / cpi, chunk = block %
+// cpi); otherwise chunks[] and descs[] list them.
+template <bool UNIFORM>
 __global__ __launch_bounds__(kPlainLanes) void k_plain_mul(const u32 *__restrict__ F,
                                                            const u32 *__restrict__ G,
                                                            u32 *__restrict__ H,
                                                            const MulDesc *__restrict__ descs,
                                                            const PlainChunk *__restrict__ chunks,
-                                                           PlainMod m) {
+                                                           MulDesc one, int cpi, PlainMod m) {
     __shared__ __align__(16) u32 sa[kPlainALds];
     __shared__ __align__(16) u32 sb[kPlainJT];
-    const PlainChunk ch = chunks[blockIdx.x];
-    const MulDesc d = descs[ch.item];
+    PlainChunk ch;
+    MulDesc d;
+    if constexpr (UNIFORM) {
+        const int64_t item = blockIdx.x / cpi;
+        ch.item = 0;
+        ch.c0 = (int32_t)(blockIdx.x % cpi) * kPlainChunk;
+        d = one;
+        d.f = item * one.f;
+        d.g = item * one.g;
+        d.h = item * one.h;
+    } else {
+        ch = chunks[blockIdx.x];
+        d = descs[ch.item];
+    }
     const bool sw = d.logn != 0;
     const u32 *A = sw ? G + d.g : F + d.f;
     const u32 *B = sw ? F + d.f : G + d.g;
@@ -139,14 +133,6 @@ __global__ __launch_bounds__(kPlainLanes) void k_plain_mul(const u32 *__restrict
     }
 }
 
-static PlainMod plain_mod(u32 p) {
-    PlainMod m;
-    m.p = p;
-    m.np = cm_host_npinv(p);
-    m.c32 = (u32)(((u64)1 << 32) % p);
-    m.r2 = (u32)((unsigned __int128)((u64)m.c32 * m.c32) % p);
-    return m;
-}
 
 hipError_t launch_plain_list(const MulDesc *descs, int64_t count, const u32 *F, const u32 *G,
                              u32 *H, u32 p, hipStream_t s) {
@@ -171,7 +157,23 @@ hipError_t launch_plain_list(const MulDesc *descs, int64_t count, const u32 *F, 
     const PlainMod m = plain_mod(p);
     ProfScope ps(kProfPlainMul, s);
     const dim3 g((unsigned)chunks.size());
-    hipLaunchKernelGGL(k_plain_mul, g, dim3(kPlainLanes), 0, s, F, G, H, d_descs, d_chunks, m);
+    hipLaunchKernelGGL(k_plain_mul<false>, g, dim3(kPlainLanes), 0, s, F, G, H, d_descs,
+                       d_chunks, MulDesc{}, 0, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_strided(const u32 *F, int64_t fs, int lf, const u32 *G, int64_t gs,
+                                int lg, u32 *H, int64_t hs, int lh, int count, u32 p,
+                                hipStream_t s) {
+    if (count <= 0 || lh <= 0) return hipSuccess;
+    if (lf < 1 || lg < 1 || (int64_t)lh > (int64_t)lf + lg - 1) return hipErrorInvalidValue;
+    const int cpi = (lh + kPlainChunk - 1) / kPlainChunk;
+    if ((int64_t)count * cpi > INT32_MAX) return hipErrorInvalidValue;
+    const MulDesc one{fs, gs, hs, lf, lg, lh, lg > lf ? 1 : 0};
+    const PlainMod m = plain_mod(p);
+    ProfScope ps(kProfPlainMul, s);
+    hipLaunchKernelGGL(k_plain_mul<true>, dim3((unsigned)(count * cpi)), dim3(kPlainLanes), 0, s,
+                       F, G, H, nullptr, nullptr, one, cpi, m);
     return hipGetLastError();
 }
 
