@@ -366,6 +366,53 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv_d(
         cm_col_dit_last<TL, true>(lds, Y, n2, lim, lvi, md);
 }
 
+// Column passes of the multiply for columns of at most 16 words (one
+// round): each thread transforms one whole column in registers, straight
+// from global memory and back, and a wave reads or writes 64 consecutive
+// columns of a row (256-byte segments).  No LDS.  256 columns per workgroup.
+constexpr int kColRegThreads = 256;
+
+template <int LOGL1, bool TOPZERO>
+__global__ __launch_bounds__(kColRegThreads) void k_col_fwd_r(
+    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv, Mod md) {
+    static_assert(LOGL1 >= 1 && LOGL1 <= 4, "one round");
+    constexpr int L = 1 << LOGL1, M = TOPZERO ? L / 2 : L;
+    const int y = blockIdx.y;
+    const ColSet &S = y < cnt0 ? s0 : s1;
+    const int64_t a = y < cnt0 ? y : y - cnt0;
+    const int64_t n2 = (int64_t)1 << logn2;
+    const int64_t col = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kColRegThreads + threadIdx.x;
+    const u32 *X = S.in + a * S.in_stride + col;
+    u32 *Y = S.out + a * S.out_stride + col;
+    const int64_t lim = S.in_len - col;
+    u32 v[1][L];
+#pragma unroll
+    for (int m = 0; m < M; ++m) v[0][m] = (int64_t)m * n2 < lim ? X[(int64_t)m * n2] : 0u;
+    cm_dif_stages<LOGL1, LOGL1, 0, 0, 1, TOPZERO>(v, 0, lv, md);
+#pragma unroll
+    for (int m = 0; m < L; ++m) Y[(int64_t)m * n2] = v[0][m];
+}
+
+template <int LOGL1>
+__global__ __launch_bounds__(kColRegThreads) void k_col_inv_r(
+    ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
+    static_assert(LOGL1 >= 1 && LOGL1 <= 4, "one round");
+    constexpr int L = 1 << LOGL1;
+    const int64_t a = blockIdx.y;
+    const int64_t n2 = (int64_t)1 << logn2;
+    const int64_t col = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kColRegThreads + threadIdx.x;
+    const u32 *X = S.in + a * S.in_stride + col;
+    u32 *Y = S.out + a * S.out_stride + col;
+    const int64_t lim = S.out_len - col;
+    u32 v[1][L];
+#pragma unroll
+    for (int m = 0; m < L; ++m) v[0][m] = X[(int64_t)m * n2];
+    cm_dit_stages<LOGL1, LOGL1, 0, 0, 1>(v, 0, lvi, md);
+#pragma unroll
+    for (int m = 0; m < L; ++m)
+        if ((int64_t)m * n2 < lim) Y[(int64_t)m * n2] = v[0][m];
+}
+
 // Row pass of the four-step multiply: row r of F and G (r = a * n1 + j, the
 // row holding frequency k1 = brev(j)); elements i = t + T m are multiplied by
 // the outer twiddle w_n^(k1 i) = rb[k1 T + t] * rc[16 k1 + m] on the way in,
@@ -642,6 +689,18 @@ static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cn
                            dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
         return hipGetLastError();
     }
+    if constexpr (LOGL1 <= 4) {
+        if (g_col_direct && logn2 >= 8) {
+            const dim3 g((unsigned)(((int64_t)1 << logn2) / kColRegThreads), cnt0 + cnt1);
+            if (topzero)
+                hipLaunchKernelGGL((k_col_fwd_r<LOGL1, true>), g, dim3(kColRegThreads), 0, s, s0,
+                                   s1, cnt0, logn2, lv, md);
+            else
+                hipLaunchKernelGGL((k_col_fwd_r<LOGL1, false>), g, dim3(kColRegThreads), 0, s, s0,
+                                   s1, cnt0, logn2, lv, md);
+            return hipGetLastError();
+        }
+    }
     if constexpr (TL::NR >= 2) {
         if (g_col_direct) {
             if (topzero)
@@ -688,6 +747,14 @@ static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *f
     using TL = Tile<LOGL1, C, true>;
     const int nb = (int)(((int64_t)1 << logn2) / C);
     ProfScope ps(kProfColInv, s);
+    if constexpr (LOGL1 <= 4) {
+        if (g_col_direct && logn2 >= 8) {
+            hipLaunchKernelGGL((k_col_inv_r<LOGL1>),
+                               dim3((unsigned)(((int64_t)1 << logn2) / kColRegThreads), cnt),
+                               dim3(kColRegThreads), 0, s, S, logn2, fam->lvi, mod_of(fam));
+            return hipGetLastError();
+        }
+    }
     if constexpr (TL::NR >= 2) {
         if (g_col_direct) {
             hipLaunchKernelGGL((k_col_inv_d<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
