@@ -126,6 +126,30 @@ def test_workload2_full_size(cm, orc):
     assert (H == orc.ntt_polymul(F, G, p)).all()
 
 
+@pytest.mark.parametrize("logn", [25, 26])
+def test_largest_transforms_evaluation_identity(cm, orc, logn):
+    """2^25- and 2^26-point multiplies (2^14-word rows, the largest supported
+    size) checked at random points: H(r) = F(r) G(r) mod p, plus a product
+    with operands longer than half the transform (no zero upper half)."""
+    rng = np.random.default_rng(logn)
+    p = P2
+    n = 1 << logn
+    for lf, lg in [(n // 2, n // 2), (n // 2 + 12345, n // 2 - 12346)]:
+        F, G = rand_poly(rng, lf, p), rand_poly(rng, lg, p)
+        dF, dG = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(G)
+        dH = cm.DeviceArray(lf + lg - 1)
+        try:
+            cm.fftmul_uni_dev(lf + lg - 2, dH, lf - 1, dF, lg - 1, dG, p)
+            H = dH.download()
+        finally:
+            for d in (dF, dG, dH):
+                d.free()
+        pts = np.array([3, 987654321, p - 2], np.int32)
+        hv, fv, gv = orc.poly_eval(H, pts, p), orc.poly_eval(F, pts, p), orc.poly_eval(G, pts, p)
+        for i in range(len(pts)):
+            assert int(hv[i]) == int(fv[i]) * int(gv[i]) % p, (logn, lf, lg, int(pts[i]))
+
+
 @pytest.mark.parametrize("p", [P1, P2, 257])
 def test_forward_transform_raw_outputs(cm, orc, p):
     """cumodp_fft_uni returns the reference's natural-order DFT for the root w."""
