@@ -63,13 +63,23 @@ __global__ void k_inv_base(const u32 *__restrict__ M, int64_t nodes, PlainMod pm
     }
 }
 
-// f(j) = rev(M_(i+1)(j)) mod x^(2h): f_t = M_(2h - t), t < 2h (node length 2h + 1)
-__global__ void k_rev_node(const u32 *__restrict__ M, int h2, int64_t nodes,
-                           u32 *__restrict__ f) {
-    const int64_t total = nodes * h2;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+// Reversed nodes of every level the subinverse tree needs, in one launch:
+// for level l = first + blockIdx.y (node length h2 + 1, h2 = 2^l),
+// f(j) = rev(M_l(j)) mod x^h2, f_t = M_(h2 - t) for t < h2, stored at
+// out + blockIdx.y n (each level's reversed nodes take n words).
+struct LevelOffsets {
+    int64_t off[32];   // word offset of level l in the tree
+};
+
+__global__ void k_rev_nodes(const u32 *__restrict__ tree, LevelOffsets lo, int first, int64_t n,
+                            u32 *__restrict__ out) {
+    const int l = first + (int)blockIdx.y;
+    const int h2 = 1 << l;
+    const u32 *M = tree + lo.off[l];
+    u32 *f = out + (int64_t)blockIdx.y * n;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
          g += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = g >> __builtin_ctz((unsigned)h2);   // h2 is a power of two
+        const int64_t j = g >> l;
         const int t = (int)(g & (h2 - 1));
         f[g] = M[j * (h2 + 1) + (h2 - t)];
     }
@@ -109,15 +119,23 @@ __global__ void k_reverse(const u32 *__restrict__ in, int h, int64_t nodes,
     }
 }
 
-// R(j)_t = A(j/2)_t - qm(j)_t, t < h
+// R(j)_t = A(j/2)_t - qm(j)_t, t < h.  With next != nullptr, also the next
+// level's reversed top halves (k_div_rev_top of R at h/2): both children c
+// of node j get next(c)_t' = R(j)_(h-1-t') for t' < h/2.
 __global__ void k_div_finish(const u32 *__restrict__ A, const u32 *__restrict__ qm, int h,
-                             int64_t nodes, u32 p, u32 *__restrict__ R) {
+                             int64_t nodes, u32 p, u32 *__restrict__ R, u32 *__restrict__ next) {
     const int64_t total = nodes * h;
+    const int hh = h >> 1;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
          g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = g >> __builtin_ctz((unsigned)h);
         const int t = (int)(g & (h - 1));
-        R[g] = cm_sub(A[(j >> 1) * (2 * h) + t], qm[g], p);
+        const u32 r = cm_sub(A[(j >> 1) * (2 * h) + t], qm[g], p);
+        R[g] = r;
+        if (next && t >= hh) {
+            next[(2 * j) * hh + (h - 1 - t)] = r;
+            next[(2 * j + 1) * hh + (h - 1 - t)] = r;
+        }
     }
 }
 
@@ -140,7 +158,8 @@ __global__ void k_horner(const u32 *__restrict__ R, int h, const u32 *__restrict
 int64_t eval_scratch_words(int k) {
     const int64_t n = (int64_t)1 << k;
     const int L0 = k < kEvalBase ? k : kEvalBase;
-    return tree_level_offset(k, k + 1) + (int64_t)(k - L0 + 1) * n + 5 * n;
+    const int nrev = k - 1 - L0 > 0 ? k - 1 - L0 : 0;          // reversed-node levels
+    return tree_level_offset(k, k + 1) + (int64_t)(k - L0 + 1) * n + 5 * n + (int64_t)nrev * n;
 }
 
 hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u32 *values,
@@ -151,6 +170,7 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     u32 *inv = tree + tree_level_offset(k, k + 1);           // levels L0 .. k, n words each
     u32 *R0 = inv + (int64_t)(k - L0 + 1) * n, *R1 = R0 + n;
     u32 *T1 = R1 + n, *T2 = T1 + n, *T3 = T2 + n;
+    u32 *rev = T3 + n;                                       // levels L0+1 .. k-1, n words each
     auto inv_at = [&](int i) { return inv + (int64_t)(i - L0) * n; };
     auto node = [&](int i) { return tree + tree_level_offset(k, i); };
 
@@ -177,6 +197,13 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
                                n >> L0, plain_mod(p), inv_at(L0));
         e = hipGetLastError();
     }
+    if (e == hipSuccess && k - 1 > L0) {
+        LevelOffsets lo{};
+        for (int l = 0; l <= k && l < 32; ++l) lo.off[l] = tree_level_offset(k, l);
+        hipLaunchKernelGGL(k_rev_nodes, dim3(eblocks(n), k - 1 - L0), dim3(256), 0, s, tree, lo,
+                           L0 + 1, n, rev);
+        e = hipGetLastError();
+    }
     for (int i = L0; i + 1 <= k - 1 && e == hipSuccess; ++i) {
         const int h = 1 << i;
         const int N = (int)(n >> (i + 1));                   // nodes of level i + 1
@@ -184,11 +211,9 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         e = launch_mul_strided(i + 1, inv_at(i), 2 * h, h, inv_at(i) + h, 2 * h, h, T1, h, h, N,
                                fam, s);
         if (e != hipSuccess) break;
-        // f(j) = rev(M_(i+1)(j)) mod x^(2h)
-        hipLaunchKernelGGL(k_rev_node, dim3(eblocks((int64_t)N * 2 * h)), dim3(256), 0, s,
-                           node(i + 1), 2 * h, (int64_t)N, T2);
-        // e(j) = f(j) g0(j) mod x^(2h)
-        e = launch_mul_strided(i + 2, T2, 2 * h, 2 * h, T1, h, h, T3, 2 * h, 2 * h, N, fam, s);
+        // e(j) = f(j) g0(j) mod x^(2h), f(j) = rev(M_(i+1)(j)) mod x^(2h)
+        const u32 *f = rev + (int64_t)(i - L0) * n;
+        e = launch_mul_strided(i + 2, f, 2 * h, 2 * h, T1, h, h, T3, 2 * h, 2 * h, N, fam, s);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_two_minus, dim3(eblocks((int64_t)N * 2 * h)), dim3(256), 0, s, T3,
                            2 * h, (int64_t)N * 2 * h, p);
@@ -197,14 +222,20 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
                                fam, s);
     }
     if (e != hipSuccess) return e;
-    // ---- remainder tree: R_k = F, then levels k-1 .. L0
-    e = hipMemcpyAsync(R0, F, sizeof(u32) * (size_t)n, hipMemcpyDeviceToDevice, s);
-    u32 *A = R0, *Rn = R1;
+    // ---- remainder tree: R_k = F (read in place), then levels k-1 .. L0; the
+    // reversed top halves of a level's dividends come from the previous
+    // level's k_div_finish (from F at the first level)
+    const u32 *A = F;
+    u32 *Rn = R0;
+    if (k - 1 >= L0) {
+        const int h = 1 << (k - 1);
+        hipLaunchKernelGGL(k_div_rev_top, dim3(eblocks(n)), dim3(256), 0, s, F, h,
+                           n >> (k - 1), T1);
+        e = hipGetLastError();
+    }
     for (int i = k - 1; i >= L0 && e == hipSuccess; --i) {
         const int h = 1 << i;
         const int N = (int)(n >> i);                         // nodes of level i
-        hipLaunchKernelGGL(k_div_rev_top, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, A, h,
-                           (int64_t)N, T1);
         // qrev(j) = revtop(j) I_i(j) mod x^h
         e = launch_mul_strided(i + 1, T1, h, h, inv_at(i), h, h, T2, h, h, N, fam, s);
         if (e != hipSuccess) break;
@@ -214,9 +245,10 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         e = launch_mul_strided(i + 1, T1, h, h, node(i), h + 1, h, T2, h, h, N, fam, s);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_div_finish, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, A, T2, h,
-                           (int64_t)N, p, Rn);
+                           (int64_t)N, p, Rn, i > L0 ? T1 : nullptr);
         e = hipGetLastError();
-        u32 *t = A; A = Rn; Rn = t;
+        A = Rn;
+        Rn = Rn == R0 ? R1 : R0;
     }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_horner, dim3(eblocks(n)), dim3(256), 0, s, A, 1 << L0, pts, n,

@@ -38,22 +38,9 @@ def is_prime(n):
     return True
 
 
-def four_terms(p):
-    c = (1 << 32) % p
-    return (2**32 - 1) * c + 2**32 - 1 + 4 * (p - 1) ** 2 < 2**64
-
-
 P_TWO = next(q for q in range((1 << 30) + 1, (1 << 30) + 1000, 2) if is_prime(q))
 P_TOP = next(q for q in range((1 << 31) - 1, (1 << 31) - 1000, -2) if is_prime(q))
 MODULI = [2013265921, 469762049, P_TOP, P_TWO, 1000003, 65537, 257, 3]
-
-
-def test_four_products_between_folds_fit_every_modulus():
-    """The bound the kernel relies on, over the moduli where it is tightest."""
-    for p in (P_TOP, P_TWO, 2013265921, 3, 5):
-        assert four_terms(p), p
-    for q in range((1 << 31) - 1, (1 << 31) - 200000, -2):
-        assert four_terms(q)
 
 
 def _rand(rng, n, p):
