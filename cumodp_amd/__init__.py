@@ -152,6 +152,9 @@ def lib():
         L.cumodp_tune_set_list_plain.argtypes = [s]
         L.cumodp_tune_set_list_plain.restype = err
         L.cumodp_tune_get_list_plain.restype = s
+        L.cumodp_tune_set_zmul_merge.argtypes = [s]
+        L.cumodp_tune_set_zmul_merge.restype = err
+        L.cumodp_tune_get_zmul_merge.restype = s
         L.cumodp_tune_set_col_tile.argtypes = [s]
         L.cumodp_tune_set_col_tile.restype = err
         L.cumodp_tune_get_col_tile.restype = s

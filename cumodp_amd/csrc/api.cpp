@@ -760,6 +760,14 @@ cumodp_err cumodp_tune_set_list_tile(int words) {
 
 int cumodp_tune_get_list_tile(void) { return list_tile_words(); }
 
+cumodp_err cumodp_tune_set_zmul_merge(int merge) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return set_zmul_merge(merge) ? CUMODP_SUCCESS
+                                 : fail(CUMODP_ASSUMPTION_ERROR, "zmul merge: 0 or 1");
+}
+
+int cumodp_tune_get_zmul_merge(void) { return zmul_merge(); }
+
 cumodp_err cumodp_tune_set_list_plain(int cutoff) {
     std::lock_guard<std::mutex> lk(api_mutex());
     return set_list_plain_cutoff(cutoff) ? CUMODP_SUCCESS

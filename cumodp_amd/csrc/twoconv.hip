@@ -362,6 +362,20 @@ hipError_t launch_two_convolution(const u64 *a, int64_t na, int wa, const u64 *b
 // launches, in chunks whose scratch stays under kTcChunkBytes.
 static constexpr size_t kTcChunkBytes = (size_t)1 << 30;
 
+// Class merging of the batched multiply: 0 keeps one class per exact plan;
+// 1 gives every item of one y size the plan of the widest coefficients of
+// that size (exact for all of them: K M covers every width and the bound
+// depends on the y size only), trading transform work for fewer launches.
+static int g_tc_merge = 0;
+
+bool set_zmul_merge(int m) {
+    if (m < 0 || m > 1) return false;
+    g_tc_merge = m;
+    return true;
+}
+
+int zmul_merge() { return g_tc_merge; }
+
 hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const u64 *A,
                                        const u64 *B, u64 *C, hipStream_t s, int *err) {
     *err = 0;
@@ -370,13 +384,23 @@ hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const
         std::vector<int64_t> idx;
     };
     std::vector<Cls> classes;
-    for (int64_t i = 0; i < count; ++i) {
-        const TcItem &it = items[i];
+    auto y_size = [](const TcItem &it) {
         int ey = 0;
         while (((int64_t)1 << ey) < it.na + it.nb - 1) ++ey;
-        if (ey < 4) ey = 4;
+        return ey < 4 ? 4 : ey;
+    };
+    int widest[64] = {0};   // merged classes: widest coefficients per y size
+    if (g_tc_merge)
+        for (int64_t i = 0; i < count; ++i) {
+            const int ey = y_size(items[i]);
+            if (ey < 64) widest[ey] = std::max(widest[ey], (int)items[i].bits);
+        }
+    for (int64_t i = 0; i < count; ++i) {
+        const TcItem &it = items[i];
+        const int ey = y_size(it);
+        const int bits = (g_tc_merge && ey < 64) ? widest[ey] : it.bits;
         TcPlan pl;
-        if (!tc_class_plan(it.bits, ey, pl) || pl.lk2 > kMaxLdsLog) {
+        if (!tc_class_plan(bits, ey, pl) || pl.lk2 > kMaxLdsLog) {
             *err = 1;
             return hipSuccess;
         }

@@ -190,6 +190,12 @@ CUMODP_API int cumodp_tune_get_list_tile(void);
  * 256; 0 sends every product to the NTT; at most 1024) */
 CUMODP_API cumodp_err cumodp_tune_set_list_plain(int cutoff);
 CUMODP_API int cumodp_tune_get_list_plain(void);
+/* cumodp_two_convolution_list_dev groups products by plan: 0 (default) one
+ * group per exact plan; 1 one group per transform length, with the plan of
+ * the widest coefficients of that length (same results, fewer launches, more
+ * transform work for the narrower products) */
+CUMODP_API cumodp_err cumodp_tune_set_zmul_merge(int merge);
+CUMODP_API int cumodp_tune_get_zmul_merge(void);
 /* words per workgroup tile of the four-step multiply's column passes: 16384
  * (the default; 1024 threads) or 8192 (512 threads, more workgroups) */
 CUMODP_API cumodp_err cumodp_tune_set_col_tile(int words);

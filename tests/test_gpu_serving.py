@@ -74,6 +74,30 @@ def test_list_mixed_sizes_exact_and_alone(cm, orc):
         assert (alone == got[r]).all(), r
 
 
+def test_list_merged_classes_same_products(cm, orc):
+    """Grouping every width of one transform length under the widest plan
+    (cumodp_tune_set_zmul_merge(1)) gives the same words as exact plans."""
+    from cumodp_amd.workload import SplitMix64
+    rng = SplitMix64(502)
+    sizes = [(100, 120), (90, 140), (200, 50), (1000, 900), (1100, 800), (3, 4), (60, 70)]
+    bitsv = [16, 64, 48, 1, 63, 32, 20]
+    A = [_signed(rng, na, b) for (na, _), b in zip(sizes, bitsv)]
+    B = [_signed(rng, nb, b) for (_, nb), b in zip(sizes, bitsv)]
+    L = cm.lib()
+    before = L.cumodp_tune_get_zmul_merge()
+    try:
+        assert L.cumodp_tune_set_zmul_merge(0) == 0
+        exact = _run_list(cm, A, B, bitsv)
+        assert L.cumodp_tune_set_zmul_merge(1) == 0
+        merged = _run_list(cm, A, B, bitsv)
+        assert L.cumodp_tune_set_zmul_merge(2) == cm.CUMODP_ASSUMPTION_ERROR
+    finally:
+        L.cumodp_tune_set_zmul_merge(before)
+    for r in range(len(A)):
+        assert (merged[r] == exact[r]).all(), r
+        assert (exact[r] == orc.zmul_i64(A[r], B[r])[:, :exact[r].shape[1]]).all(), r
+
+
 def test_list_rejects_narrow_output(cm):
     items = (cm.ZmulItem * 1)()
     items[0].na = items[0].nb = 4
