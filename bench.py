@@ -449,6 +449,15 @@ def copy_calibration(nbytes=256 << 20):
     log(f"bench: copied {nbytes} bytes three times")
 
 
+SECONDARY = {
+    "list_multiply": bench_list_multiply,
+    "plain_crossover": bench_plain_crossover,
+    "subproduct_tree": bench_tree,
+    "fast_evaluation": bench_eval,
+    "two_convolution": bench_two_convolution,
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -465,6 +474,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (reference for "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--only", choices=sorted(SECONDARY) + ["serving"],
+                    help="run one secondary line alone and print it (for profiling)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -473,6 +484,15 @@ def main():
         print(f"bench.py measures one GPU only (the path is single-GPU); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        import cumodp_amd as cm
+        cpu = not args.no_cpu_baseline
+        if args.only == "serving":
+            r = bench_serving(cm, cpu, args.serving_requests)
+        else:
+            r = SECONDARY[args.only](cm, cpu)
+        print(json.dumps({args.only: r}), flush=True)
+        return
 
     import cumodp_amd as cm
     L = cm.lib()
@@ -610,11 +630,7 @@ def main():
     if not args.primary_only:
         sec = {}
         cpu = not args.no_cpu_baseline
-        for key, fn in (("list_multiply", bench_list_multiply),
-                        ("plain_crossover", bench_plain_crossover),
-                        ("subproduct_tree", bench_tree),
-                        ("fast_evaluation", bench_eval),
-                        ("two_convolution", bench_two_convolution)):
+        for key, fn in SECONDARY.items():
             log(f"bench: {key}")
             sec[key] = fn(cm, cpu)
             log(f"bench: {key}: {sec[key]}")
