@@ -331,6 +331,28 @@ CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, Mod md) {
 // register.  No LDS staging pass and no barrier precede the first round or
 // follow the last.
 
+// Word `off` of a row whose base is uniform, addressed as base + 4 off with a
+// 32-bit byte offset.  off < 2^30 (tiles of transforms up to 2^26 words).
+CM_DEV u32 cm_ld_row(const u32 *base, u32 off) {
+    return *(const u32 *)((const char *)base + (off << 2));
+}
+CM_DEV void cm_st_row(u32 *base, u32 off, u32 v) { *(u32 *)((char *)base + (off << 2)) = v; }
+
+// Raw buffer view of a tile: the tile base in a scalar resource, each access
+// = per-lane byte offset (vector register) + uniform row byte offset (scalar
+// register), so no vector instruction computes an address.  No range limit
+// (every access stays inside the arrays the caller passed); word 3 is the
+// raw-buffer format of the gfx9 family.
+CM_DEV __amdgpu_buffer_rsrc_t cm_tile_rsrc(const u32 *base) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, -1, 0x00020000);
+}
+CM_DEV u32 cm_buf_ld(__amdgpu_buffer_rsrc_t r, u32 off_bytes, u32 row_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off_bytes, (int)row_bytes, 0);
+}
+CM_DEV void cm_buf_st(__amdgpu_buffer_rsrc_t r, u32 off_bytes, u32 row_bytes, u32 v) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)off_bytes, (int)row_bytes, 0);
+}
+
 // Forward round 0 from global memory into LDS.  CHECK: elements at or beyond
 // lim (relative to X) read as zero.  TOPZERO: rows >= L/2 are zero and not
 // read.
@@ -344,11 +366,17 @@ CM_DEV void cm_col_dif_first(u32 *lds, const u32 *X, int64_t n2, int64_t lim, co
         typename RD::G g(threadIdx.x + q * TL::T);
         const u32 off = (u32)g.i0 * (u32)n2 + (u32)g.c;
         u32 v[1][1 << R];
+        if constexpr (CHECK) {
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
-            const u32 *Xm = X + rowoff;
-            v[0][m] = (!CHECK || rowoff + off < lim) ? Xm[off] : 0u;
+            for (int m = 0; m < M; ++m) {
+                const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
+                v[0][m] = rowoff + off < lim ? cm_ld_row(X + rowoff, off) : 0u;
+            }
+        } else {
+            const auto rx = cm_tile_rsrc(X);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+                v[0][m] = cm_buf_ld(rx, off << 2, (u32)((m << RD::DLOG) * n2) << 2);
         }
         cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1, TOPZERO>(v, g.gi_lo, lv, md);
 #pragma unroll
@@ -361,6 +389,7 @@ template <class TL>
 CM_DEV void cm_col_dif_last(const u32 *lds, u32 *Y, int64_t n2, const u32 *lv, Mod md) {
     using RD = Round<TL, TL::NR - 1>;
     constexpr int R = RD::R;
+    const auto ry = cm_tile_rsrc(Y);
 #pragma unroll
     for (int q = 0; q < RD::GPT; ++q) {
         typename RD::G g(threadIdx.x + q * TL::T);
@@ -370,7 +399,8 @@ CM_DEV void cm_col_dif_last(const u32 *lds, u32 *Y, int64_t n2, const u32 *lv, M
         for (int m = 0; m < (1 << R); ++m) v[0][m] = lds[g.a(m)];
         cm_dif_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(v, g.gi_lo, lv, md);
 #pragma unroll
-        for (int m = 0; m < (1 << R); ++m) (Y + (int64_t)(m << RD::DLOG) * n2)[off] = v[0][m];
+        for (int m = 0; m < (1 << R); ++m)
+            cm_buf_st(ry, off << 2, (u32)((m << RD::DLOG) * n2) << 2, v[0][m]);
     }
 }
 
@@ -379,13 +409,15 @@ template <class TL>
 CM_DEV void cm_col_dit_first(u32 *lds, const u32 *X, int64_t n2, const u32 *lvi, Mod md) {
     using RD = Round<TL, TL::NR - 1>;
     constexpr int R = RD::R;
+    const auto rx = cm_tile_rsrc(X);
 #pragma unroll
     for (int q = 0; q < RD::GPT; ++q) {
         typename RD::G g(threadIdx.x + q * TL::T);
         const u32 off = (u32)g.i0 * (u32)n2 + (u32)g.c;
         u32 v[1][1 << R];
 #pragma unroll
-        for (int m = 0; m < (1 << R); ++m) v[0][m] = (X + (int64_t)(m << RD::DLOG) * n2)[off];
+        for (int m = 0; m < (1 << R); ++m)
+            v[0][m] = cm_buf_ld(rx, off << 2, (u32)((m << RD::DLOG) * n2) << 2);
         cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(v, g.gi_lo, lvi, md);
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) lds[g.a(m)] = v[0][m];
@@ -407,10 +439,17 @@ CM_DEV void cm_col_dit_last(const u32 *lds, u32 *Y, int64_t n2, int64_t lim, con
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) v[0][m] = lds[g.a(m)];
         cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(v, g.gi_lo, lvi, md);
+        if constexpr (CHECK) {
 #pragma unroll
-        for (int m = 0; m < (1 << R); ++m) {
-            const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
-            if (!CHECK || rowoff + off < lim) (Y + rowoff)[off] = v[0][m];
+            for (int m = 0; m < (1 << R); ++m) {
+                const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
+                if (rowoff + off < lim) cm_st_row(Y + rowoff, off, v[0][m]);
+            }
+        } else {
+            const auto ry = cm_tile_rsrc(Y);
+#pragma unroll
+            for (int m = 0; m < (1 << R); ++m)
+                cm_buf_st(ry, off << 2, (u32)((m << RD::DLOG) * n2) << 2, v[0][m]);
         }
     }
 }
