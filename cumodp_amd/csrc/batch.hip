@@ -52,6 +52,28 @@ CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const M
     // of a thread are issued before the first LDS store
     constexpr int PER = TL::WORDS / kMixT;
     u32 fa[PER], ga[PER];
+    if constexpr (Round<TL, 0>::R == 4 && Round<TL, 0>::GPT == 1 && TL::NR >= 2) {
+        // thread t loads its group of the first forward and the last inverse
+        // round (product c, elements gi + r 2^(LOGL-4)), which run on the
+        // registers
+        constexpr int S = LOGL - 4;
+        const int c = threadIdx.x >> S, gi = threadIdx.x & ((1 << S) - 1);
+        const bool live = c < cnt;
+        const MulDesc dc = live ? d[c] : MulDesc{};
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int i = gi + (r << S);
+            fa[r] = live && i < dc.lf ? F[dc.f + i] : 0u;
+            ga[r] = live && i < dc.lg ? G[dc.g + i] : 0u;
+        }
+        cm_row_multiply<TL>(A, B, fa, ga, lv, lvi, md, sc);
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int i = gi + (r << S);
+            if (live && i < dc.lh) H[dc.h + i] = fa[r];
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         const int idx = threadIdx.x + r * kMixT, c = idx >> LOGL, i = idx & (L - 1);

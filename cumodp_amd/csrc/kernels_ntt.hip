@@ -46,68 +46,92 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
     u32 *A = lds, *B = lds + TL::LDS_WORDS;
     const int L = TL::L;
 
-    int64_t off_f[C], off_g[C], off_h[C];
-    int lf[C], lg[C], lh[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
+    if constexpr (Round<TL, 0>::R == 4 && Round<TL, 0>::GPT == 1 && TL::NR >= 2) {
+        // thread t's product is c = t >> (LOGL-4); it loads its group of the
+        // first forward and the last inverse round (elements gi + r 2^(LOGL-4)),
+        // which run on the registers
+        constexpr int S = LOGL - 4;
+        constexpr int PER = TL::WORDS / TL::T;
+        const int c = threadIdx.x >> S, gi = threadIdx.x & ((1 << S) - 1);
         const int d = blockIdx.x * C + c;
-        if (DESC) {
-            if (d < ndesc) {
-                const MulDesc dd = descs ? descs[d] : one;
-                off_f[c] = dd.f; off_g[c] = dd.g; off_h[c] = dd.h;
-                lf[c] = dd.lf; lg[c] = dd.lg; lh[c] = dd.lh;
+        MulDesc my{};
+        if (d < ndesc) {
+            if (DESC) {
+                my = descs ? descs[d] : one;
             } else {
-                off_f[c] = off_g[c] = off_h[c] = 0;
-                lf[c] = lg[c] = lh[c] = 0;
+                // product d at d times the strides one.f / one.g / one.h
+                my = one;
+                my.f = (int64_t)d * one.f; my.g = (int64_t)d * one.g; my.h = (int64_t)d * one.h;
             }
-        } else {
-            // product d at d times the strides one.f / one.g / one.h, with
-            // the lengths one.lf / one.lg / one.lh
-            off_f[c] = (int64_t)d * one.f; off_g[c] = (int64_t)d * one.g;
-            off_h[c] = (int64_t)d * one.h;
-            lf[c] = d < ndesc ? one.lf : 0; lg[c] = d < ndesc ? one.lg : 0;
-            lh[c] = d < ndesc ? one.lh : 0;
         }
-    }
-    // all loads of a thread are issued before the first LDS store
-    constexpr int PER = TL::WORDS / TL::T;
-    u32 fa[PER], ga[PER];
+        u32 fa[PER], ga[PER];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
-        fa[r] = i < lf[c] ? F[off_f[c] + i] : 0u;
-        ga[r] = i < lg[c] ? G[off_g[c] + i] : 0u;
-    }
-    if constexpr (C == 1 && Round<TL, 0>::R == 4 && Round<TL, 0>::GPT == 1 && TL::NR >= 2) {
-        // one product per workgroup: the loaded words t + T r are thread t's
-        // group of the first forward and the last inverse round, which run
-        // on the registers
+        for (int r = 0; r < PER; ++r) {
+            const int i = gi + (r << S);
+            fa[r] = i < my.lf ? F[my.f + i] : 0u;
+            ga[r] = i < my.lg ? G[my.g + i] : 0u;
+        }
         cm_row_multiply<TL>(A, B, fa, ga, lv, lvi, md, sc);
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
-            const int i = threadIdx.x + r * TL::T;
-            if (i < lh[0]) H[off_h[0] + i] = fa[r];
+            const int i = gi + (r << S);
+            if (i < my.lh) H[my.h + i] = fa[r];
         }
-        for (int i = L + threadIdx.x; i < lh[0]; i += TL::T) H[off_h[0] + i] = 0u;
+        // words beyond the transform size (lh > L) are zero
+        for (int i = L + gi; i < my.lh; i += 1 << S) H[my.h + i] = 0u;
         return;
-    }
+    } else {
+
+        int64_t off_f[C], off_g[C], off_h[C];
+        int lf[C], lg[C], lh[C];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
-        A[TL::addr(c, i)] = fa[r];
-        B[TL::addr(c, i)] = ga[r];
-    }
-    __syncthreads();
-    cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
+        for (int c = 0; c < C; ++c) {
+            const int d = blockIdx.x * C + c;
+            if (DESC) {
+                if (d < ndesc) {
+                    const MulDesc dd = descs ? descs[d] : one;
+                    off_f[c] = dd.f; off_g[c] = dd.g; off_h[c] = dd.h;
+                    lf[c] = dd.lf; lg[c] = dd.lg; lh[c] = dd.lh;
+                } else {
+                    off_f[c] = off_g[c] = off_h[c] = 0;
+                    lf[c] = lg[c] = lh[c] = 0;
+                }
+            } else {
+                // product d at d times the strides one.f / one.g / one.h, with
+                // the lengths one.lf / one.lg / one.lh
+                off_f[c] = (int64_t)d * one.f; off_g[c] = (int64_t)d * one.g;
+                off_h[c] = (int64_t)d * one.h;
+                lf[c] = d < ndesc ? one.lf : 0; lg[c] = d < ndesc ? one.lg : 0;
+                lh[c] = d < ndesc ? one.lh : 0;
+            }
+        }
+        // all loads of a thread are issued before the first LDS store
+        constexpr int PER = TL::WORDS / TL::T;
+        u32 fa[PER], ga[PER];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
-        if (i < lh[c]) H[off_h[c] + i] = A[TL::addr(c, i)];
-    }
-    // words beyond the transform size (lh > L) are zero
+        for (int r = 0; r < PER; ++r) {
+            const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+            fa[r] = i < lf[c] ? F[off_f[c] + i] : 0u;
+            ga[r] = i < lg[c] ? G[off_g[c] + i] : 0u;
+        }
 #pragma unroll
-    for (int c = 0; c < C; ++c)
-        for (int i = L + threadIdx.x; i < lh[c]; i += TL::T) H[off_h[c] + i] = 0u;
+        for (int r = 0; r < PER; ++r) {
+            const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+            A[TL::addr(c, i)] = fa[r];
+            B[TL::addr(c, i)] = ga[r];
+        }
+        __syncthreads();
+        cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
+#pragma unroll
+        for (int r = 0; r < PER; ++r) {
+            const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
+            if (i < lh[c]) H[off_h[c] + i] = A[TL::addr(c, i)];
+        }
+        // words beyond the transform size (lh > L) are zero
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            for (int i = L + threadIdx.x; i < lh[c]; i += TL::T) H[off_h[c] + i] = 0u;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -437,11 +461,12 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     const u32 *Gr = G + r * TL::L;
     const int t = threadIdx.x;
     const u32 b = rb[(int64_t)k1 * T + t];
+    const auto rf = cm_tile_rsrc(Fr), rg = cm_tile_rsrc(Gr);
     u32 fa[16], ga[16];
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        fa[m] = Fr[t + T * m];
-        ga[m] = Gr[t + T * m];
+        fa[m] = cm_buf_ld(rf, (u32)t << 2, (u32)(T * m) << 2);
+        ga[m] = cm_buf_ld(rg, (u32)t << 2, (u32)(T * m) << 2);
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
@@ -456,7 +481,8 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const u32 c = rci[k1 * 16 + m];
-        Fr[t + T * m] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), bi, md.p, md.np);
+        cm_buf_st(rf, (u32)t << 2, (u32)(T * m) << 2,
+                  cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), bi, md.p, md.np));
     }
 }
 

@@ -43,6 +43,16 @@ struct Mod {
     u32 p, np;
 };
 
+// Twiddle w = tab[base + mm + lane] of a level table (base and mm uniform,
+// lane per thread; lane == 0 when the index is uniform).  Plain loads: a raw
+// buffer form (uniform part as the scalar offset) measured slower here,
+// 80 against 75 us on the forward column pass of the 2^24 multiply.
+template <bool UNIFORM>
+CM_DEV u32 cm_twiddle(const u32 *tab, int base_mm, int lane) {
+    if constexpr (UNIFORM) return tab[base_mm];
+    else return tab[base_mm + lane];
+}
+
 // (x, y) -> (x + y, (x - y) w): the difference enters the product unreduced
 // (x - y + p < 2p, and 2p * w < p 2^32).
 CM_DEV void cm_dif_bfly(u32 &x, u32 &y, u32 w, Mod md) {
@@ -170,7 +180,7 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict
 #pragma unroll
                     for (int a = 0; a < NV; ++a) v[a][m + h] = v[a][m];
                 } else {
-                    const u32 w = lv[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+                    const u32 w = cm_twiddle<DLOG == 0>(lv, base + mm, gi_lo);
 #pragma unroll
                     for (int a = 0; a < NV; ++a) v[a][m + h] = cm_mul_tw(v[a][m], w, md.p, md.np);
                 }
@@ -180,7 +190,7 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict
 #pragma unroll
                 for (int a = 0; a < NV; ++a) cm_dif_bfly1(v[a][m], v[a][m + h], md);
             } else {
-                const u32 w = lv[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+                const u32 w = cm_twiddle<DLOG == 0>(lv, base + mm, gi_lo);
 #pragma unroll
                 for (int a = 0; a < NV; ++a) cm_dif_bfly(v[a][m], v[a][m + h], w, md);
             }
@@ -203,7 +213,7 @@ CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict
 #pragma unroll
                 for (int a = 0; a < NV; ++a) cm_dit_bfly1(v[a][m], v[a][m + h], md);
             } else {
-                const u32 w = lvi[base + (DLOG == 0 ? 0 : gi_lo) + mm];
+                const u32 w = cm_twiddle<DLOG == 0>(lvi, base + mm, gi_lo);
 #pragma unroll
                 for (int a = 0; a < NV; ++a) cm_dit_bfly(v[a][m], v[a][m + h], w, md);
             }
@@ -498,17 +508,19 @@ CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod 
     cm_dit_rounds<TL, TL::NR - 2, 0, 1>(a1, lvi, md);
 }
 
-// The same multiply for one row tile (C == 1, 16 elements per thread) whose
-// round-0 group, elements t + T m of thread t, is in registers on entry (fa
-// for A, ga for B) and whose product's round-0 group is left in fa: the
-// first forward and the last inverse round skip the LDS staging pass and its
-// barrier.  sc as in cm_tile_multiply.
+// The same multiply for a tile of row vectors (16 elements per thread, first
+// round radix-16) whose round-0 groups are in registers on entry (fa for A,
+// ga for B) and whose product's round-0 groups are left in fa: the first
+// forward and the last inverse round skip the LDS staging pass and its
+// barrier.  Thread t's group is vector c = t >> (LOGL-4), elements
+// gi + m 2^(LOGL-4), gi = t mod 2^(LOGL-4) (elements t + T m when C == 1).
+// sc as in cm_tile_multiply.
 template <class TL>
 CM_DEV void cm_row_multiply(u32 *A, u32 *B, u32 (&fa)[16], const u32 (&ga)[16], const u32 *lv,
                             const u32 *lvi, Mod md, u32 sc) {
     using R0 = Round<TL, 0>;
-    static_assert(TL::C == 1 && R0::R == 4 && R0::GPT == 1 && TL::NR >= 2,
-                  "row tiles of 16 elements per thread");
+    static_assert(!TL::COLS && R0::R == 4 && R0::GPT == 1 && TL::NR >= 2,
+                  "row tiles of 16 elements per thread, radix-16 first round");
     const typename R0::G g(threadIdx.x);
     u32 v[2][16];
 #pragma unroll
