@@ -341,10 +341,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv(
 // The same forward and inverse column passes of the multiply with the first
 // and last rounds reading and writing global memory directly (no LDS staging
 // pass, two barriers fewer; any alignment).  Tiles of two or more rounds.
-template <int LOGL1, int C, bool TOPZERO>
+template <int LOGL1, int C, bool TOPZERO, class MD>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
 __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd_d(
-    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv, Mod md) {
+    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv, MD md) {
     using TL = Tile<LOGL1, C, true>;
     static_assert(TL::NR >= 2, "direct column pass needs two rounds");
     __shared__ __align__(16) u32 lds[TL::LDS_WORDS];
@@ -367,10 +367,10 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd_d(
     cm_col_dif_last<TL>(lds, Y, n2, lv, md);
 }
 
-template <int LOGL1, int C>
+template <int LOGL1, int C, class MD>
 __global__ __launch_bounds__((Tile<LOGL1, C, true>::T))
 __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv_d(
-    ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
+    ColSet S, int logn2, const u32 *__restrict__ lvi, MD md) {
     using TL = Tile<LOGL1, C, true>;
     static_assert(TL::NR >= 2, "direct column pass needs two rounds");
     __shared__ __align__(16) u32 lds[TL::LDS_WORDS];
@@ -396,9 +396,9 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_inv_d(
 // columns of a row (256-byte segments).  No LDS.  256 columns per workgroup.
 constexpr int kColRegThreads = 256;
 
-template <int LOGL1, bool TOPZERO>
+template <int LOGL1, bool TOPZERO, class MD>
 __global__ __launch_bounds__(kColRegThreads) void k_col_fwd_r(
-    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv, Mod md) {
+    ColSet s0, ColSet s1, int cnt0, int logn2, const u32 *__restrict__ lv, MD md) {
     static_assert(LOGL1 >= 1 && LOGL1 <= 4, "one round");
     constexpr int L = 1 << LOGL1, M = TOPZERO ? L / 2 : L;
     const int y = blockIdx.y;
@@ -417,9 +417,9 @@ __global__ __launch_bounds__(kColRegThreads) void k_col_fwd_r(
     for (int m = 0; m < L; ++m) Y[(int64_t)m * n2] = v[0][m];
 }
 
-template <int LOGL1>
+template <int LOGL1, class MD>
 __global__ __launch_bounds__(kColRegThreads) void k_col_inv_r(
-    ColSet S, int logn2, const u32 *__restrict__ lvi, Mod md) {
+    ColSet S, int logn2, const u32 *__restrict__ lvi, MD md) {
     static_assert(LOGL1 >= 1 && LOGL1 <= 4, "one round");
     constexpr int L = 1 << LOGL1;
     const int64_t a = blockIdx.y;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kColRegThreads) void k_col_inv_r(
     cm_dit_stages<LOGL1, LOGL1, 0, 0, 1>(v, 0, lvi, md);
 #pragma unroll
     for (int m = 0; m < L; ++m)
-        if ((int64_t)m * n2 < lim) Y[(int64_t)m * n2] = v[0][m];
+        if ((int64_t)m * n2 < lim) Y[(int64_t)m * n2] = cm_canon(v[0][m], md);
 }
 
 // Row pass of the four-step multiply: row r of F and G (r = a * n1 + j, the
@@ -444,12 +444,12 @@ __global__ __launch_bounds__(kColRegThreads) void k_col_inv_r(
 // inverse outer twiddle is applied on the way out.  In place on F.
 // Occupancy target: 2 workgroups per CU for 512-thread rows (4 waves/EU),
 // 4 for 256-thread rows.
-template <int LOGL2>
+template <int LOGL2, class MD>
 __global__ __launch_bounds__((Tile<LOGL2, 1, false>::T))
 __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     u32 *F, const u32 *__restrict__ G, int logn1, const u32 *__restrict__ rb,
     const u32 *__restrict__ rc, const u32 *__restrict__ rbi, const u32 *__restrict__ rci,
-    const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md, u32 sc) {
+    const u32 *__restrict__ lv, const u32 *__restrict__ lvi, MD md, u32 sc) {
     using TL = Tile<LOGL2, 1, false>;
     constexpr int T = TL::T;
     static_assert(TL::L == 16 * T, "row pass assumes 16 elements per thread");
@@ -471,8 +471,8 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const u32 c = rc[k1 * 16 + m];
-        fa[m] = cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md.p, md.np);
-        ga[m] = cm_mul_tw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md.p, md.np);
+        fa[m] = cm_mulw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md);
+        ga[m] = cm_mulw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md);
     }
     // elements t + T m are thread t's group of the first forward and the last
     // inverse round: those rounds run on the registers
@@ -482,7 +482,7 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
     for (int m = 0; m < 16; ++m) {
         const u32 c = rci[k1 * 16 + m];
         cm_buf_st(rf, (u32)t << 2, (u32)(T * m) << 2,
-                  cm_mul_tw(cm_mont_lazy(fa[m], c, md.p, md.np), bi, md.p, md.np));
+                  cm_mulw(cm_mont_lazy(fa[m], c, md.p, md.np), bi, md));
     }
 }
 
@@ -700,9 +700,11 @@ static bool g_col_direct = [] {
     return !(e && *e == '0');
 }();
 
-template <int LOGL1, int WORDS>
+// MD = ModLazy (multiply only, values in [0, 2p) between the passes) runs
+// the direct and register kernels only.
+template <int LOGL1, int WORDS, class MD>
 static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
-                            const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
+                            const u32 *lv, const u32 *olo, const u32 *ohi, MD md,
                             hipStream_t s) {
     constexpr int C = col_tile<LOGL1, WORDS>();
     using TL = Tile<LOGL1, C, true>;
@@ -710,35 +712,39 @@ static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     ProfScope ps(kProfColFwd, s);
     const int64_t half = (int64_t)1 << (LOGL1 + logn2 - 1);
     const bool topzero = LOGL1 >= 2 && s0.in_len <= half && (cnt1 == 0 || s1.in_len <= half);
-    if (olo) {
-        hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true, false>), dim3(nb, cnt0 + cnt1),
-                           dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
-        return hipGetLastError();
+    if constexpr (!MD::LAZY) {
+        if (olo) {
+            hipLaunchKernelGGL((k_col_fwd<LOGL1, C, true, false>), dim3(nb, cnt0 + cnt1),
+                               dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
+            return hipGetLastError();
+        }
     }
     if constexpr (LOGL1 <= 4) {
         if (g_col_direct && logn2 >= 8) {
             const dim3 g((unsigned)(((int64_t)1 << logn2) / kColRegThreads), cnt0 + cnt1);
             if (topzero)
-                hipLaunchKernelGGL((k_col_fwd_r<LOGL1, true>), g, dim3(kColRegThreads), 0, s, s0,
-                                   s1, cnt0, logn2, lv, md);
+                hipLaunchKernelGGL((k_col_fwd_r<LOGL1, true, MD>), g, dim3(kColRegThreads), 0, s,
+                                   s0, s1, cnt0, logn2, lv, md);
             else
-                hipLaunchKernelGGL((k_col_fwd_r<LOGL1, false>), g, dim3(kColRegThreads), 0, s, s0,
-                                   s1, cnt0, logn2, lv, md);
+                hipLaunchKernelGGL((k_col_fwd_r<LOGL1, false, MD>), g, dim3(kColRegThreads), 0, s,
+                                   s0, s1, cnt0, logn2, lv, md);
             return hipGetLastError();
         }
     }
     if constexpr (TL::NR >= 2) {
         if (g_col_direct) {
             if (topzero)
-                hipLaunchKernelGGL((k_col_fwd_d<LOGL1, C, true>), dim3(nb, cnt0 + cnt1),
+                hipLaunchKernelGGL((k_col_fwd_d<LOGL1, C, true, MD>), dim3(nb, cnt0 + cnt1),
                                    dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, md);
             else
-                hipLaunchKernelGGL((k_col_fwd_d<LOGL1, C, false>), dim3(nb, cnt0 + cnt1),
+                hipLaunchKernelGGL((k_col_fwd_d<LOGL1, C, false, MD>), dim3(nb, cnt0 + cnt1),
                                    dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, md);
             return hipGetLastError();
         }
     }
-    if constexpr (C % 4 == 0) {
+    if constexpr (MD::LAZY) {
+        return hipErrorInvalidValue;
+    } else if constexpr (C % 4 == 0) {
         if (vec_ok(s0) && (cnt1 == 0 || vec_ok(s1)) && topzero)
             hipLaunchKernelGGL((k_col_fwd<LOGL1, C, false, true, true>), dim3(nb, cnt0 + cnt1),
                                dim3(TL::T), 0, s, s0, s1, cnt0, logn2, lv, olo, ohi, md);
@@ -755,9 +761,9 @@ static hipError_t col_fwd_w(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     return hipGetLastError();
 }
 
-template <int LOGL1>
+template <int LOGL1, class MD>
 static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cnt1, int logn2,
-                            const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
+                            const u32 *lv, const u32 *olo, const u32 *ohi, MD md,
                             hipStream_t s) {
     // the 8192-word tile serves the multiply only (stand-alone transforms
     // pass outer twiddles)
@@ -766,8 +772,8 @@ static hipError_t col_fwd_t(const ColSet &s0, const ColSet &s1, int cnt0, int cn
     return col_fwd_w<LOGL1, 16384>(s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, md, s);
 }
 
-template <int LOGL1, int WORDS>
-static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *fam,
+template <int LOGL1, int WORDS, class MD>
+static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *fam, MD md,
                             hipStream_t s) {
     constexpr int C = col_tile<LOGL1, WORDS>();
     using TL = Tile<LOGL1, C, true>;
@@ -775,36 +781,41 @@ static hipError_t col_inv_w(const ColSet &S, int cnt, int logn2, const Family *f
     ProfScope ps(kProfColInv, s);
     if constexpr (LOGL1 <= 4) {
         if (g_col_direct && logn2 >= 8) {
-            hipLaunchKernelGGL((k_col_inv_r<LOGL1>),
+            hipLaunchKernelGGL((k_col_inv_r<LOGL1, MD>),
                                dim3((unsigned)(((int64_t)1 << logn2) / kColRegThreads), cnt),
-                               dim3(kColRegThreads), 0, s, S, logn2, fam->lvi, mod_of(fam));
+                               dim3(kColRegThreads), 0, s, S, logn2, fam->lvi, md);
             return hipGetLastError();
         }
     }
     if constexpr (TL::NR >= 2) {
         if (g_col_direct) {
-            hipLaunchKernelGGL((k_col_inv_d<LOGL1, C>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
-                               logn2, fam->lvi, mod_of(fam));
+            hipLaunchKernelGGL((k_col_inv_d<LOGL1, C, MD>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
+                               logn2, fam->lvi, md);
             return hipGetLastError();
         }
     }
-    if constexpr (C % 4 == 0) {
-        if (vec_ok(S)) {
-            hipLaunchKernelGGL((k_col_inv<LOGL1, C, true>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
-                               logn2, fam->lvi, mod_of(fam));
-            return hipGetLastError();
+    if constexpr (MD::LAZY) {
+        return hipErrorInvalidValue;
+    } else {
+        if constexpr (C % 4 == 0) {
+            if (vec_ok(S)) {
+                hipLaunchKernelGGL((k_col_inv<LOGL1, C, true>), dim3(nb, cnt), dim3(TL::T), 0, s,
+                                   S, logn2, fam->lvi, md);
+                return hipGetLastError();
+            }
         }
+        hipLaunchKernelGGL((k_col_inv<LOGL1, C, false>), dim3(nb, cnt), dim3(TL::T), 0, s, S,
+                           logn2, fam->lvi, md);
+        return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_col_inv<LOGL1, C, false>), dim3(nb, cnt), dim3(TL::T), 0, s, S, logn2,
-                       fam->lvi, mod_of(fam));
-    return hipGetLastError();
 }
 
-template <int LOGL1>
-static hipError_t col_inv_t(const ColSet &S, int cnt, int logn2, const Family *fam,
+template <int LOGL1, class MD>
+static hipError_t col_inv_t(const ColSet &S, int cnt, int logn2, const Family *fam, MD md,
                             hipStream_t s) {
-    if (g_col_words == 8192 && LOGL1 >= 9) return col_inv_w<LOGL1, 8192>(S, cnt, logn2, fam, s);
-    return col_inv_w<LOGL1, 16384>(S, cnt, logn2, fam, s);
+    if (g_col_words == 8192 && LOGL1 >= 9)
+        return col_inv_w<LOGL1, 8192>(S, cnt, logn2, fam, md, s);
+    return col_inv_w<LOGL1, 16384>(S, cnt, logn2, fam, md, s);
 }
 
 template <int LOGL2>
@@ -831,28 +842,26 @@ static hipError_t row_out_t(const u32 *X, u32 *Y, int cnt, int64_t stride, int l
     }                                                                                    \
     return hipErrorInvalidValue;
 
+template <class MD>
 static hipError_t col_fwd(int k1, const ColSet &s0, const ColSet &s1, int cnt0, int cnt1,
-                          int logn2, const u32 *lv, const u32 *olo, const u32 *ohi, Mod md,
+                          int logn2, const u32 *lv, const u32 *olo, const u32 *ohi, MD md,
                           hipStream_t s) {
     CM_K1_CASES(col_fwd_t, s0, s1, cnt0, cnt1, logn2, lv, olo, ohi, md, s)
 }
 
-static hipError_t col_inv(int k1, const ColSet &S, int cnt, int logn2, const Family *fam,
+template <class MD>
+static hipError_t col_inv(int k1, const ColSet &S, int cnt, int logn2, const Family *fam, MD md,
                           hipStream_t s) {
-    CM_K1_CASES(col_inv_t, S, cnt, logn2, fam, s)
+    CM_K1_CASES(col_inv_t, S, cnt, logn2, fam, md, s)
 }
 
-hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, hipStream_t s) {
-    if (k <= kMaxLdsLog || k > kMaxLog || k > fam->K) return hipErrorInvalidValue;
-    int k1, k2;
-    four_step_split(k, k1, k2);
-    const Outer *o = outer_for(fam, k);
-    if (!o) return hipErrorOutOfMemory;
-    const int64_t n = (int64_t)1 << k;
+template <class MD>
+static hipError_t mul_four_step_t(int k1, int k2, const FourStepIO &io, const Outer *o,
+                                  const Family *fam, MD md, hipStream_t s) {
+    const int64_t n = (int64_t)1 << (k1 + k2);
     ColSet sf{io.f_in, io.f_ws, io.in_stride, io.f_len, n, n};
     ColSet sg{io.g_in, io.g_ws, io.g_stride < 0 ? io.in_stride : io.g_stride, io.g_len, n, n};
-    hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr,
-                           mod_of(fam), s);
+    hipError_t e = col_fwd(k1, sf, sg, io.cnt, io.cnt, k2, fam->lv, nullptr, nullptr, md, s);
     if (e != hipSuccess) return e;
     // row pass: one row of F and G per workgroup (2^k1 * cnt rows); the
     // pointwise scale 2^32 / n is carried by the inverse row twiddles
@@ -860,10 +869,10 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     const u32 sc = 0u;
     {
         ProfScope ps(kProfRowMul, s);
-#define CM_ROW(K2)                                                                         \
-    hipLaunchKernelGGL((k_row_mul<K2>), dim3(nrows), dim3(Tile<K2, 1, false>::T), 0, s,    \
-                       io.f_ws, io.g_ws, k1, o->rb_f, o->rc_f, o->rb_i, o->rc_i, fam->lv,  \
-                       fam->lvi, mod_of(fam), sc)
+#define CM_ROW(K2)                                                                          \
+    hipLaunchKernelGGL((k_row_mul<K2, MD>), dim3(nrows), dim3(Tile<K2, 1, false>::T), 0, s, \
+                       io.f_ws, io.g_ws, k1, o->rb_f, o->rc_f, o->rb_i, o->rc_i, fam->lv,   \
+                       fam->lvi, md, sc)
         if (k2 == 12) CM_ROW(12);
         else if (k2 == 13) CM_ROW(13);
         else CM_ROW(14);
@@ -872,7 +881,21 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     ColSet sh{io.f_ws, io.h_out, n, n, io.out_stride, io.h_len};
-    return col_inv(k1, sh, io.cnt, k2, fam, s);
+    return col_inv(k1, sh, io.cnt, k2, fam, md, s);
+}
+
+// The multiply keeps its values in [0, 2p) between butterflies and passes
+// (ModLazy) when p < 2^30 and the direct column kernels run; the last
+// inverse round writes canonical residues.
+hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, hipStream_t s) {
+    if (k <= kMaxLdsLog || k > kMaxLog || k > fam->K) return hipErrorInvalidValue;
+    int k1, k2;
+    four_step_split(k, k1, k2);
+    const Outer *o = outer_for(fam, k);
+    if (!o) return hipErrorOutOfMemory;
+    if (g_col_direct && fam->p < (1u << 30))
+        return mul_four_step_t(k1, k2, io, o, fam, ModLazy{fam->p, fam->np}, s);
+    return mul_four_step_t(k1, k2, io, o, fam, mod_of(fam), s);
 }
 
 template <int LOGL>

@@ -38,10 +38,43 @@ CM_DEV int cm_pad(int i) { return i + (i >> 4); }
 
 CM_DEV u32 cm_brev(u32 x, int bits) { return __brev(x) >> (32 - bits); }
 
-// Modulus and -p^-1 mod 2^32 for the lazy Montgomery products.
+// Modulus and -p^-1 mod 2^32 for the lazy Montgomery products; every
+// butterfly returns canonical residues ([0, p)).  Any odd p < 2^31.
 struct Mod {
     u32 p, np;
+    static constexpr bool LAZY = false;
 };
+
+// The same for p < 2^30 with values kept in [0, 2p) (Harvey's lazy
+// butterflies, here on Montgomery products): a sum or difference of two such
+// values is below 4p < 2^32 and is brought back below 2p with one
+// conditional subtraction of 2p, and the Montgomery product of a value below
+// 4p by a twiddle below p is below p 2^32, so it lands in [0, 2p) with no
+// final subtraction.  Two instructions fewer per twiddled butterfly; results
+// that leave a transform pass through cm_canon.
+struct ModLazy {
+    u32 p, np;
+    static constexpr bool LAZY = true;
+};
+
+// the range bound c of the values between butterflies: p or 2p
+template <class MD>
+CM_DEV u32 cm_bound(MD md) { return MD::LAZY ? md.p << 1 : md.p; }
+
+// a * w for a twiddle or constant w < p in Montgomery form and a < 2c:
+// canonical for Mod, in [0, 2p) for ModLazy
+template <class MD>
+CM_DEV u32 cm_mulw(u32 a, u32 w, MD md) {
+    if constexpr (MD::LAZY) return cm_mont_lazy(a, w, md.p, md.np);
+    else return cm_mul_tw(a, w, md.p, md.np);
+}
+
+// canonical residue of a value in [0, c)
+template <class MD>
+CM_DEV u32 cm_canon(u32 a, MD md) {
+    if constexpr (MD::LAZY) return cm_reduce2p(a, md.p);
+    else return a;
+}
 
 // Twiddle w = tab[base + mm + lane] of a level table (base and mm uniform,
 // lane per thread; lane == 0 when the index is uniform).  Plain loads: a raw
@@ -53,32 +86,41 @@ CM_DEV u32 cm_twiddle(const u32 *tab, int base_mm, int lane) {
     else return tab[base_mm + lane];
 }
 
-// (x, y) -> (x + y, (x - y) w): the difference enters the product unreduced
-// (x - y + p < 2p, and 2p * w < p 2^32).
-CM_DEV void cm_dif_bfly(u32 &x, u32 &y, u32 w, Mod md) {
-    u32 s = cm_add(x, y, md.p);
-    u32 d = x - y + md.p;              // (0, 2p)
-    y = cm_mul_tw(d, w, md.p, md.np);
+// (x, y) -> (x + y, (x - y) w) on values in [0, c): the difference enters
+// the product unreduced (x - y + c < 2c, and 2c w < p 2^32).  cm_reduce2p and
+// cm_sub are exact for any bound c with 2c < 2^32.
+template <class MD>
+CM_DEV void cm_dif_bfly(u32 &x, u32 &y, u32 w, MD md) {
+    const u32 c = cm_bound(md);
+    u32 s = cm_reduce2p(x + y, c);
+    u32 d = x - y + c;                 // (0, 2c)
+    y = cm_mulw(d, w, md);
     x = s;
 }
 
-CM_DEV void cm_dif_bfly1(u32 &x, u32 &y, Mod md) {
-    u32 s = cm_add(x, y, md.p);
-    y = cm_sub(x, y, md.p);
+template <class MD>
+CM_DEV void cm_dif_bfly1(u32 &x, u32 &y, MD md) {
+    const u32 c = cm_bound(md);
+    u32 s = cm_reduce2p(x + y, c);
+    y = cm_sub(x, y, c);
     x = s;
 }
 
 // (a, b) -> (a + b w, a - b w)
-CM_DEV void cm_dit_bfly(u32 &a, u32 &b, u32 w, Mod md) {
-    u32 t = cm_mul_tw(b, w, md.p, md.np);
-    b = cm_sub(a, t, md.p);
-    a = cm_add(a, t, md.p);
+template <class MD>
+CM_DEV void cm_dit_bfly(u32 &a, u32 &b, u32 w, MD md) {
+    const u32 c = cm_bound(md);
+    u32 t = cm_mulw(b, w, md);
+    b = cm_sub(a, t, c);
+    a = cm_reduce2p(a + t, c);
 }
 
-CM_DEV void cm_dit_bfly1(u32 &a, u32 &b, Mod md) {
+template <class MD>
+CM_DEV void cm_dit_bfly1(u32 &a, u32 &b, MD md) {
+    const u32 c = cm_bound(md);
     u32 t = b;
-    b = cm_sub(a, t, md.p);
-    a = cm_add(a, t, md.p);
+    b = cm_sub(a, t, c);
+    a = cm_reduce2p(a + t, c);
 }
 
 template <int LOGL_, int C_, bool COLS_, int T_ = (C_ << LOGL_) / 16>
@@ -165,8 +207,8 @@ CM_DEV uint4 cm_lds_get4(const u32 *lds, int idx) {
 // groups that share their positions (and so their twiddles).  TOPZERO: the
 // upper half of the group (m >= 2^(R-1)) holds zeros, so the first stage is
 // (x, 0) -> (x, x w) and those inputs are never read.
-template <int LOGL, int R, int S0, int DLOG, int NV, bool TOPZERO = false>
-CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lv, Mod md) {
+template <int LOGL, int R, int S0, int DLOG, int NV, bool TOPZERO = false, class MD>
+CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lv, MD md) {
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         const int h = 1 << (R - 1 - t);
@@ -182,7 +224,7 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict
                 } else {
                     const u32 w = cm_twiddle<DLOG == 0>(lv, base + mm, gi_lo);
 #pragma unroll
-                    for (int a = 0; a < NV; ++a) v[a][m + h] = cm_mul_tw(v[a][m], w, md.p, md.np);
+                    for (int a = 0; a < NV; ++a) v[a][m + h] = cm_mulw(v[a][m], w, md);
                 }
                 continue;
             }
@@ -199,8 +241,8 @@ CM_DEV void cm_dif_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict
 }
 
 // Inverse of cm_dif_stages with the inverse level table.
-template <int LOGL, int R, int S0, int DLOG, int NV>
-CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lvi, Mod md) {
+template <int LOGL, int R, int S0, int DLOG, int NV, class MD>
+CM_DEV void cm_dit_stages(u32 (&v)[NV][1 << R], int gi_lo, const u32 *__restrict__ lvi, MD md) {
 #pragma unroll
     for (int t = R - 1; t >= 0; --t) {
         const int h = 1 << (R - 1 - t);
@@ -257,8 +299,8 @@ struct Round {
     }
     // full forward round on NV tiles, LDS -> LDS; TOPZERO (first round
     // only): the upper half of every vector is zero and is not read
-    template <int NV, bool TOPZERO = false>
-    static CM_DEV void dif(u32 *const (&lds)[NV], const u32 *lv, Mod md) {
+    template <int NV, bool TOPZERO = false, class MD>
+    static CM_DEV void dif(u32 *const (&lds)[NV], const u32 *lv, MD md) {
         static_assert(!TOPZERO || S0 == 0, "zero upper half only in the first round");
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
@@ -277,8 +319,8 @@ struct Round {
         }
     }
     // full inverse round on NV tiles, LDS -> LDS
-    template <int NV>
-    static CM_DEV void dit(u32 *const (&lds)[NV], const u32 *lvi, Mod md) {
+    template <int NV, class MD>
+    static CM_DEV void dit(u32 *const (&lds)[NV], const u32 *lvi, MD md) {
 #pragma unroll
         for (int q = 0; q < GPT; ++q) {
             G g(threadIdx.x + q * TL::T);
@@ -291,8 +333,8 @@ struct Round {
 };
 
 // Forward rounds [J, J1) on NV tiles with a barrier after each.
-template <class TL, int J, int J1, int NV>
-CM_DEV void cm_dif_rounds(u32 *const (&lds)[NV], const u32 *lv, Mod md) {
+template <class TL, int J, int J1, int NV, class MD>
+CM_DEV void cm_dif_rounds(u32 *const (&lds)[NV], const u32 *lv, MD md) {
     if constexpr (J < J1) {
         Round<TL, J>::template dif<NV>(lds, lv, md);
         __syncthreads();
@@ -301,8 +343,8 @@ CM_DEV void cm_dif_rounds(u32 *const (&lds)[NV], const u32 *lv, Mod md) {
 }
 
 // Inverse rounds J, J-1, ..., J0 (J >= J0) with a barrier after each.
-template <class TL, int J, int J0, int NV>
-CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const u32 *lvi, Mod md) {
+template <class TL, int J, int J0, int NV, class MD>
+CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const u32 *lvi, MD md) {
     if constexpr (J >= J0) {
         Round<TL, J>::template dit<NV>(lds, lvi, md);
         __syncthreads();
@@ -312,8 +354,8 @@ CM_DEV void cm_dit_rounds(u32 *const (&lds)[NV], const u32 *lvi, Mod md) {
 
 // Full forward transform of every vector of one tile whose upper halves are
 // zero (and were not stored to LDS).
-template <class TL>
-CM_DEV void cm_tile_dif_topzero(u32 *lds, const u32 *lv, Mod md) {
+template <class TL, class MD>
+CM_DEV void cm_tile_dif_topzero(u32 *lds, const u32 *lv, MD md) {
     u32 *const t[1] = {lds};
     Round<TL, 0>::template dif<1, true>(t, lv, md);
     __syncthreads();
@@ -321,14 +363,14 @@ CM_DEV void cm_tile_dif_topzero(u32 *lds, const u32 *lv, Mod md) {
 }
 
 // Full forward transform of every vector of one tile (bit-reversed result).
-template <class TL>
-CM_DEV void cm_tile_dif(u32 *lds, const u32 *lv, Mod md) {
+template <class TL, class MD>
+CM_DEV void cm_tile_dif(u32 *lds, const u32 *lv, MD md) {
     u32 *const t[1] = {lds};
     cm_dif_rounds<TL, 0, TL::NR, 1>(t, lv, md);
 }
 
-template <class TL>
-CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, Mod md) {
+template <class TL, class MD>
+CM_DEV void cm_tile_dit(u32 *lds, const u32 *lvi, MD md) {
     u32 *const t[1] = {lds};
     cm_dit_rounds<TL, TL::NR - 1, 0, 1>(t, lvi, md);
 }
@@ -366,9 +408,9 @@ CM_DEV void cm_buf_st(__amdgpu_buffer_rsrc_t r, u32 off_bytes, u32 row_bytes, u3
 // Forward round 0 from global memory into LDS.  CHECK: elements at or beyond
 // lim (relative to X) read as zero.  TOPZERO: rows >= L/2 are zero and not
 // read.
-template <class TL, bool TOPZERO, bool CHECK>
+template <class TL, bool TOPZERO, bool CHECK, class MD>
 CM_DEV void cm_col_dif_first(u32 *lds, const u32 *X, int64_t n2, int64_t lim, const u32 *lv,
-                             Mod md) {
+                             MD md) {
     using RD = Round<TL, 0>;
     constexpr int R = RD::R, M = TOPZERO ? (1 << (R - 1)) : (1 << R);
 #pragma unroll
@@ -395,8 +437,8 @@ CM_DEV void cm_col_dif_first(u32 *lds, const u32 *X, int64_t n2, int64_t lim, co
 }
 
 // Last forward round from LDS straight to global memory (all rows).
-template <class TL>
-CM_DEV void cm_col_dif_last(const u32 *lds, u32 *Y, int64_t n2, const u32 *lv, Mod md) {
+template <class TL, class MD>
+CM_DEV void cm_col_dif_last(const u32 *lds, u32 *Y, int64_t n2, const u32 *lv, MD md) {
     using RD = Round<TL, TL::NR - 1>;
     constexpr int R = RD::R;
     const auto ry = cm_tile_rsrc(Y);
@@ -415,8 +457,8 @@ CM_DEV void cm_col_dif_last(const u32 *lds, u32 *Y, int64_t n2, const u32 *lv, M
 }
 
 // First inverse round (round NR-1) from global memory into LDS (all rows).
-template <class TL>
-CM_DEV void cm_col_dit_first(u32 *lds, const u32 *X, int64_t n2, const u32 *lvi, Mod md) {
+template <class TL, class MD>
+CM_DEV void cm_col_dit_first(u32 *lds, const u32 *X, int64_t n2, const u32 *lvi, MD md) {
     using RD = Round<TL, TL::NR - 1>;
     constexpr int R = RD::R;
     const auto rx = cm_tile_rsrc(X);
@@ -434,11 +476,12 @@ CM_DEV void cm_col_dit_first(u32 *lds, const u32 *X, int64_t n2, const u32 *lvi,
     }
 }
 
-// Last inverse round (round 0) from LDS straight to global memory.  CHECK:
-// elements at or beyond lim (relative to Y) are not stored.
-template <class TL, bool CHECK>
+// Last inverse round (round 0) from LDS straight to global memory, canonical
+// residues out.  CHECK: elements at or beyond lim (relative to Y) are not
+// stored.
+template <class TL, bool CHECK, class MD>
 CM_DEV void cm_col_dit_last(const u32 *lds, u32 *Y, int64_t n2, int64_t lim, const u32 *lvi,
-                            Mod md) {
+                            MD md) {
     using RD = Round<TL, 0>;
     constexpr int R = RD::R;
 #pragma unroll
@@ -453,13 +496,13 @@ CM_DEV void cm_col_dit_last(const u32 *lds, u32 *Y, int64_t n2, int64_t lim, con
 #pragma unroll
             for (int m = 0; m < (1 << R); ++m) {
                 const int64_t rowoff = (int64_t)(m << RD::DLOG) * n2;
-                if (rowoff + off < lim) cm_st_row(Y + rowoff, off, v[0][m]);
+                if (rowoff + off < lim) cm_st_row(Y + rowoff, off, cm_canon(v[0][m], md));
             }
         } else {
             const auto ry = cm_tile_rsrc(Y);
 #pragma unroll
             for (int m = 0; m < (1 << R); ++m)
-                cm_buf_st(ry, off << 2, (u32)((m << RD::DLOG) * n2) << 2, v[0][m]);
+                cm_buf_st(ry, off << 2, (u32)((m << RD::DLOG) * n2) << 2, cm_canon(v[0][m], md));
         }
     }
 }
@@ -475,8 +518,8 @@ CM_DEV void cm_col_dit_last(const u32 *lds, u32 *Y, int64_t n2, int64_t lim, con
 // The middle of a tile multiply: the last forward round of A and B, the
 // pointwise product (scaled by sc as in cm_tile_multiply) and the first
 // inverse round, LDS to LDS into A, then a barrier.
-template <class TL>
-CM_DEV void cm_tile_pointwise_round(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md,
+template <class TL, class MD>
+CM_DEV void cm_tile_pointwise_round(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, MD md,
                                     u32 sc) {
     u32 *const ab[2] = {A, B};
     using RD = Round<TL, TL::NR - 1>;
@@ -490,8 +533,8 @@ CM_DEV void cm_tile_pointwise_round(u32 *A, u32 *B, const u32 *lv, const u32 *lv
         u32 w[1][1 << R];
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m)
-            w[0][m] = sc ? cm_mul_tw(cm_mont_lazy(v[0][m], v[1][m], md.p, md.np), sc, md.p, md.np)
-                         : cm_mul_tw(v[0][m], v[1][m], md.p, md.np);
+            w[0][m] = sc ? cm_mulw(cm_mont_lazy(v[0][m], v[1][m], md.p, md.np), sc, md)
+                         : cm_mulw(v[0][m], v[1][m], md);
         cm_dit_stages<TL::LOGL, R, RD::S0, RD::DLOG, 1>(w, g.gi_lo, lvi, md);
 #pragma unroll
         for (int m = 0; m < (1 << R); ++m) A[g.a(m)] = w[0][m];
@@ -499,8 +542,8 @@ CM_DEV void cm_tile_pointwise_round(u32 *A, u32 *B, const u32 *lv, const u32 *lv
     __syncthreads();
 }
 
-template <class TL>
-CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
+template <class TL, class MD>
+CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, MD md, u32 sc) {
     u32 *const ab[2] = {A, B};
     cm_dif_rounds<TL, 0, TL::NR - 1, 2>(ab, lv, md);
     cm_tile_pointwise_round<TL>(A, B, lv, lvi, md, sc);
@@ -515,9 +558,9 @@ CM_DEV void cm_tile_multiply(u32 *A, u32 *B, const u32 *lv, const u32 *lvi, Mod 
 // barrier.  Thread t's group is vector c = t >> (LOGL-4), elements
 // gi + m 2^(LOGL-4), gi = t mod 2^(LOGL-4) (elements t + T m when C == 1).
 // sc as in cm_tile_multiply.
-template <class TL>
+template <class TL, class MD>
 CM_DEV void cm_row_multiply(u32 *A, u32 *B, u32 (&fa)[16], const u32 (&ga)[16], const u32 *lv,
-                            const u32 *lvi, Mod md, u32 sc) {
+                            const u32 *lvi, MD md, u32 sc) {
     using R0 = Round<TL, 0>;
     static_assert(!TL::COLS && R0::R == 4 && R0::GPT == 1 && TL::NR >= 2,
                   "row tiles of 16 elements per thread, radix-16 first round");

@@ -113,6 +113,28 @@ def test_four_step_splits(cm, orc, k2):
             cm.lib().cumodp_tune_set_split(logn, 13 if logn >= 21 else 12)
 
 
+@pytest.mark.parametrize("p", [998244353, 1004535809, P2])
+def test_four_step_extreme_residues(cm, orc, p):
+    """Operands made of the largest residues (all p - 1, then a mix of 0, 1,
+    p - 2 and p - 1) through the four-step multiply.  Below 2^30 the
+    multiply keeps its values in [0, 2p) between butterflies and passes;
+    these inputs drive every sum and difference to the top of that range.
+    998244353 and 1004535809 are the NTT primes just below 2^30."""
+    rng = np.random.default_rng(p % 977)
+    for logn in (14, 17, 20):
+        n = 1 << logn
+        for mix in (False, True):
+            if mix:
+                vals = np.array([0, 1, p - 2, p - 1], np.int64)
+                F = vals[rng.integers(0, 4, n // 2)].astype(np.int32)
+                G = vals[rng.integers(0, 4, n // 2 - 1)].astype(np.int32)
+            else:
+                F = np.full(n // 2, p - 1, np.int64).astype(np.int32)
+                G = np.full(n // 2 - 1, p - 1, np.int64).astype(np.int32)
+            got = cm.poly_mul(F, G, p)
+            assert (got == orc.ntt_polymul(F, G, p)).all(), (p, logn, mix)
+
+
 def test_workload2_full_size(cm, orc):
     """The 2^24-point multiply mod 2013265921, bit-exact against the oracle."""
     w = cm.workload2()
