@@ -6,11 +6,16 @@ seed 2), inputs resident in HBM.  One step = one full product through
 cumodp_fftmul_uni_dev (forward transforms of both operands, pointwise
 product, inverse transform, scaling) on a single stream.
 
+W warm-up steps are followed by untimed steps for RAMP_S seconds, which
+bring the GPU from its idle clocks to its steady ones; then K steps are timed
+twice: once as a caller runs them (`value`), once with HIP events around
+every kernel launch on its stream (the per-kernel durations).
+
 Prints one JSON line: metric/value (output coefficients per second over the
 timed steps), the roofline of the dominant kernel (its algorithmic bytes per
 launch over its average duration, measured with HIP events on its stream
-during the timed region), and the CPU baseline (the oracle's serial NTT
-multiply on the same inputs, single thread, timed on this host).
+during the second timed region), and the CPU baseline (the oracle's serial
+NTT multiply on the same inputs, single thread, timed on this host).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -75,8 +80,26 @@ def cpu_baseline(F, G, p, reps=2):
     return H, t
 
 
+RAMP_S = 0.25
+
+
+def ramp(fn, sync, seconds=RAMP_S):
+    """Untimed repeats of fn() for at least `seconds` of wall time.  The GPU
+    leaves its idle clocks only after tens of milliseconds of steady work:
+    straight after an idle spell (a CPU leg, start-up) the first hundred
+    2^24 multiplies run 10-15% slower than the steady state."""
+    sync()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            fn()
+        sync()
+
+
 def timed(fn, reps, sync):
-    """Median wall time of fn() over reps runs, each bracketed by sync()."""
+    """Median wall time of fn() over reps runs, each bracketed by sync(),
+    after the clock ramp."""
+    ramp(fn, sync)
     ts = []
     for _ in range(reps):
         sync()
@@ -514,25 +537,43 @@ def main():
         cm.fftmul_uni_dev(nout - 1, dH, df, dF, dg, dG, p, stream)
 
     # the per-kernel timing events are pooled: warm-up fills the pool so the
-    # timed region creates none
+    # timed regions create none
     cm.profile_enable(True)
     for _ in range(args.warmup):
         step()
     cm.synchronize()
+    cm.profile_enable(False)
+    # then untimed steps until the GPU runs at its steady clocks
+    ramp(step, cm.synchronize)
     log("bench: warmup done")
 
-    cm.profile_reset()
+    # timed region 1: the K steps as a caller runs them (no timing events
+    # between the kernels); `value` comes from this one
     cm.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     cm.synchronize()
     t1 = time.perf_counter()
-    cm.profile_enable(False)
-    prof = cm.profile_read()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = nout / (ms_per_step * 1e-3)
-    log(f"bench: {ms_per_step:.4f} ms/step, {value:.4e} coeffs/s; kernels {prof}")
+    # timed region 2: the same K steps with a HIP event pair around every
+    # kernel launch, on the stream the kernels run on; the per-kernel
+    # durations (and the roofline's achieved rate) come from this one.  The
+    # events cost about 10 us per step, so its step time is reported apart.
+    cm.profile_enable(True)
+    cm.profile_reset()
+    cm.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    cm.synchronize()
+    t3 = time.perf_counter()
+    cm.profile_enable(False)
+    prof = cm.profile_read()
+    ms_per_step_events = (t3 - t2) * 1e3 / args.steps
+    log(f"bench: {ms_per_step:.4f} ms/step, {value:.4e} coeffs/s "
+        f"({ms_per_step_events:.4f} ms/step with kernel events); kernels {prof}")
 
     H = dH.download()
     ok_eval = all(eval_mod(H, r, p) == eval_mod(F, r, p) * eval_mod(G, r, p) % p
@@ -578,7 +619,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "clock_ramp_s": RAMP_S,
         "ms_per_step": ms_per_step,
+        "ms_per_step_with_kernel_events": ms_per_step_events,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

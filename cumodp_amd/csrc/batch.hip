@@ -42,9 +42,9 @@ struct Scales {
 template <int LOGL, int W, int T>
 using MixTile = Tile<LOGL, (W >> LOGL), false, T>;
 
-template <int LOGL, int W, int T>
+template <int LOGL, int W, int T, class MD>
 CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const MulDesc *d,
-                     int cnt, const u32 *lv, const u32 *lvi, Mod md, u32 sc) {
+                     int cnt, const u32 *lv, const u32 *lvi, MD md, u32 sc) {
     using TL = MixTile<LOGL, W, T>;
     constexpr int L = TL::L;
     constexpr int kMixT = T;
@@ -70,7 +70,7 @@ CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const M
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int i = gi + (r << S);
-            if (live && i < dc.lh) H[dc.h + i] = fa[r];
+            if (live && i < dc.lh) H[dc.h + i] = cm_canon(fa[r], md);
         }
         return;
     }
@@ -95,15 +95,15 @@ CM_DEV void mul_tile(u32 *A, u32 *B, const u32 *F, const u32 *G, u32 *H, const M
     cm_tile_multiply<TL>(A, B, lv, lvi, md, sc);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += kMixT) {
         const int c = idx >> LOGL, i = idx & (L - 1);
-        if (c < cnt && i < d[c].lh) H[d[c].h + i] = A[TL::addr(c, i)];
+        if (c < cnt && i < d[c].lh) H[d[c].h + i] = cm_canon(A[TL::addr(c, i)], md);
     }
 }
 
-template <int W, int T>
+template <int W, int T, class MD>
 __global__ __launch_bounds__(T) void k_mul_mixed(
     const u32 *__restrict__ F, const u32 *__restrict__ G, u32 *__restrict__ H,
     const MulDesc *__restrict__ descs, const TileDesc *__restrict__ tiles,
-    const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md, Scales scales) {
+    const u32 *__restrict__ lv, const u32 *__restrict__ lvi, MD md, Scales scales) {
     constexpr int LW = W + (W >> 4);
     __shared__ u32 lds[2 * LW];
     u32 *A = lds, *B = lds + LW;
@@ -184,6 +184,23 @@ bool set_list_plain_cutoff(int c) {
 
 int list_plain_cutoff() { return g_plain_cutoff; }
 
+// mixed-size tiles of mv = 0, 1, 2 (2048, 4096, 8192 words)
+template <class MD>
+static hipError_t launch_mixed(int mv, dim3 g, const u32 *F, const u32 *G, u32 *H,
+                               const MulDesc *d_descs, const TileDesc *d_tiles, const Family *fam,
+                               MD md, const Scales &sc, hipStream_t s) {
+    if (mv == 0)
+        hipLaunchKernelGGL((k_mul_mixed<2048, 128, MD>), g, dim3(128), 0, s, F, G, H, d_descs,
+                           d_tiles, fam->lv, fam->lvi, md, sc);
+    else if (mv == 1)
+        hipLaunchKernelGGL((k_mul_mixed<4096, 256, MD>), g, dim3(256), 0, s, F, G, H, d_descs,
+                           d_tiles, fam->lv, fam->lvi, md, sc);
+    else
+        hipLaunchKernelGGL((k_mul_mixed<8192, 512, MD>), g, dim3(512), 0, s, F, G, H, d_descs,
+                           d_tiles, fam->lv, fam->lvi, md, sc);
+    return hipGetLastError();
+}
+
 static hipError_t launch_list_mul_ntt(const MulDesc *descs, int64_t count, const u32 *F,
                                       const u32 *G, u32 *H, const Family *fam, hipStream_t s);
 
@@ -263,19 +280,13 @@ static hipError_t launch_list_mul_ntt(const MulDesc *descs, int64_t count, const
 
     if (!tiles.empty()) {
         ProfScope ps(kProfMulMixed, s);
-        const Mod md{fam->p, fam->np};
         const Scales sc = scale_table(fam);
         const dim3 g((unsigned)tiles.size());
-        if (mv == 0)
-            hipLaunchKernelGGL((k_mul_mixed<2048, 128>), g, dim3(128), 0, s, F, G, H, d_descs,
-                               d_tiles, fam->lv, fam->lvi, md, sc);
-        else if (mv == 1)
-            hipLaunchKernelGGL((k_mul_mixed<4096, 256>), g, dim3(256), 0, s, F, G, H, d_descs,
-                               d_tiles, fam->lv, fam->lvi, md, sc);
+        // values in [0, 2p) between butterflies when p < 2^30
+        if (fam->p < (1u << 30))
+            e = launch_mixed(mv, g, F, G, H, d_descs, d_tiles, fam, ModLazy{fam->p, fam->np}, sc, s);
         else
-            hipLaunchKernelGGL((k_mul_mixed<8192, 512>), g, dim3(512), 0, s, F, G, H, d_descs,
-                               d_tiles, fam->lv, fam->lvi, md, sc);
-        e = hipGetLastError();
+            e = launch_mixed(mv, g, F, G, H, d_descs, d_tiles, fam, Mod{fam->p, fam->np}, sc, s);
         if (e != hipSuccess) return e;
     }
     for (int64_t a = n_small_end; a < n13_end;) {

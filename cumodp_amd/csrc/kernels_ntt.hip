@@ -35,11 +35,11 @@ CM_DEV int xcd_remap(int b, int nb) {
 // words) and writes H + d * one.h (one.lh words).  H may alias F.
 // Held to 128 VGPRs so that the 2^12- and 2^13-point tiles keep as many
 // workgroups per CU as their LDS allows.
-template <int LOGL, int C, bool DESC>
+template <int LOGL, int C, bool DESC, class MD>
 __global__ __launch_bounds__((Tile<LOGL, C, false>::T))
 __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
     const u32 *F, const u32 *__restrict__ G, u32 *H, const MulDesc *__restrict__ descs,
-    int ndesc, MulDesc one, const u32 *__restrict__ lv, const u32 *__restrict__ lvi, Mod md,
+    int ndesc, MulDesc one, const u32 *__restrict__ lv, const u32 *__restrict__ lvi, MD md,
     u32 sc) {
     using TL = Tile<LOGL, C, false>;
     __shared__ u32 lds[2 * TL::LDS_WORDS];
@@ -75,7 +75,7 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int i = gi + (r << S);
-            if (i < my.lh) H[my.h + i] = fa[r];
+            if (i < my.lh) H[my.h + i] = cm_canon(fa[r], md);
         }
         // words beyond the transform size (lh > L) are zero
         for (int i = L + gi; i < my.lh; i += 1 << S) H[my.h + i] = 0u;
@@ -125,7 +125,7 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_mul_lds(
 #pragma unroll
         for (int r = 0; r < PER; ++r) {
             const int idx = threadIdx.x + r * TL::T, c = idx >> LOGL, i = idx & (L - 1);
-            if (i < lh[c]) H[off_h[c] + i] = A[TL::addr(c, i)];
+            if (i < lh[c]) H[off_h[c] + i] = cm_canon(A[TL::addr(c, i)], md);
         }
         // words beyond the transform size (lh > L) are zero
 #pragma unroll
@@ -610,6 +610,8 @@ u32 mul_scale(int logn, u32 p) {
 template <int LOGL>
 static constexpr int rows_per_wg() { return LOGL >= 10 ? 1 : (1 << (LOGL >= 10 ? 0 : 10 - LOGL)); }
 
+// Multiplies inside one workgroup keep their values in [0, 2p) between
+// butterflies when p < 2^30 (ModLazy) and store canonical residues.
 template <int LOGL, bool DESC>
 static hipError_t mul_lds_t(const u32 *F, const u32 *G, u32 *H, const MulDesc *descs, int ndesc,
                             const MulDesc &one, const Family *fam, hipStream_t s) {
@@ -617,8 +619,14 @@ static hipError_t mul_lds_t(const u32 *F, const u32 *G, u32 *H, const MulDesc *d
     using TL = Tile<LOGL, C, false>;
     const int nb = (ndesc + C - 1) / C;
     ProfScope ps(kProfMulLds, s);
-    hipLaunchKernelGGL((k_mul_lds<LOGL, C, DESC>), dim3(nb), dim3(TL::T), 0, s, F, G, H, descs,
-                       ndesc, one, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
+    if (fam->p < (1u << 30))
+        hipLaunchKernelGGL((k_mul_lds<LOGL, C, DESC, ModLazy>), dim3(nb), dim3(TL::T), 0, s, F, G,
+                           H, descs, ndesc, one, fam->lv, fam->lvi, ModLazy{fam->p, fam->np},
+                           mul_scale(LOGL, fam->p));
+    else
+        hipLaunchKernelGGL((k_mul_lds<LOGL, C, DESC, Mod>), dim3(nb), dim3(TL::T), 0, s, F, G, H,
+                           descs, ndesc, one, fam->lv, fam->lvi, mod_of(fam),
+                           mul_scale(LOGL, fam->p));
     return hipGetLastError();
 }
 
@@ -900,27 +908,15 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
 
 template <int LOGL>
 static hipError_t rows_mul_t(u32 *F, u32 *G, int rows, const Family *fam, hipStream_t s) {
-    constexpr int C = rows_per_wg<LOGL>();
-    using TL = Tile<LOGL, C, false>;
-    const int nb = (rows + C - 1) / C;
-    ProfScope ps(kProfMulLds, s);
     const int L = 1 << LOGL;
     const MulDesc rows_desc{L, L, L, L, L, L, LOGL};
-    hipLaunchKernelGGL((k_mul_lds<LOGL, C, false>), dim3(nb), dim3(TL::T), 0, s, F, G, F, nullptr,
-                       rows, rows_desc, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
-    return hipGetLastError();
+    return mul_lds_t<LOGL, false>(F, G, F, nullptr, rows, rows_desc, fam, s);
 }
 
 template <int LOGL>
 static hipError_t mul_strided_t(const u32 *F, const u32 *G, u32 *H, const MulDesc &sd, int count,
                                 const Family *fam, hipStream_t s) {
-    constexpr int C = rows_per_wg<LOGL>();
-    using TL = Tile<LOGL, C, false>;
-    const int nb = (count + C - 1) / C;
-    ProfScope ps(kProfMulLds, s);
-    hipLaunchKernelGGL((k_mul_lds<LOGL, C, false>), dim3(nb), dim3(TL::T), 0, s, F, G, H, nullptr,
-                       count, sd, fam->lv, fam->lvi, mod_of(fam), mul_scale(LOGL, fam->p));
-    return hipGetLastError();
+    return mul_lds_t<LOGL, false>(F, G, H, nullptr, count, sd, fam, s);
 }
 
 hipError_t launch_mul_strided(int logn, const u32 *F, int64_t fs, int lf, const u32 *G,

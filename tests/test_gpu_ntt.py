@@ -113,26 +113,48 @@ def test_four_step_splits(cm, orc, k2):
             cm.lib().cumodp_tune_set_split(logn, 13 if logn >= 21 else 12)
 
 
+def extreme_operands(rng, lf, lg, p, mix):
+    """All p - 1, or a random mix of 0, 1, p - 2 and p - 1."""
+    if mix:
+        vals = np.array([0, 1, p - 2, p - 1], np.int64)
+        return (vals[rng.integers(0, 4, lf)].astype(np.int32),
+                vals[rng.integers(0, 4, lg)].astype(np.int32))
+    return (np.full(lf, p - 1, np.int64).astype(np.int32),
+            np.full(lg, p - 1, np.int64).astype(np.int32))
+
+
 @pytest.mark.parametrize("p", [998244353, 1004535809, P2])
-def test_four_step_extreme_residues(cm, orc, p):
+def test_extreme_residues_multiply(cm, orc, p):
     """Operands made of the largest residues (all p - 1, then a mix of 0, 1,
-    p - 2 and p - 1) through the four-step multiply.  Below 2^30 the
-    multiply keeps its values in [0, 2p) between butterflies and passes;
-    these inputs drive every sum and difference to the top of that range.
-    998244353 and 1004535809 are the NTT primes just below 2^30."""
+    p - 2 and p - 1) through the one-workgroup multiply (2^9, 2^12), the
+    four-step multiply (2^14 .. 2^20) and the list multiply's mixed tiles.
+    Below 2^30 these kernels keep their values in [0, 2p) between
+    butterflies; these inputs drive every sum and difference to the top of
+    that range.  998244353 and 1004535809 are the NTT primes just below
+    2^30."""
     rng = np.random.default_rng(p % 977)
-    for logn in (14, 17, 20):
+    for logn in (9, 12, 14, 17, 20):
         n = 1 << logn
         for mix in (False, True):
-            if mix:
-                vals = np.array([0, 1, p - 2, p - 1], np.int64)
-                F = vals[rng.integers(0, 4, n // 2)].astype(np.int32)
-                G = vals[rng.integers(0, 4, n // 2 - 1)].astype(np.int32)
-            else:
-                F = np.full(n // 2, p - 1, np.int64).astype(np.int32)
-                G = np.full(n // 2 - 1, p - 1, np.int64).astype(np.int32)
+            F, G = extreme_operands(rng, n // 2, n // 2 - 1, p, mix)
             got = cm.poly_mul(F, G, p)
             assert (got == orc.ntt_polymul(F, G, p)).all(), (p, logn, mix)
+    degs = [(300, 20), (1000, 1000), (2047, 2048), (5000, 3000), (40, 700)]
+    pairs = [extreme_operands(rng, a + 1, b + 1, p, i % 2 == 1) for i, (a, b) in enumerate(degs)]
+    items = (cm.MulItem * len(degs))()
+    fo = go = ho = 0
+    for i, (a, b) in enumerate(degs):
+        items[i].f_off, items[i].g_off, items[i].h_off = fo, go, ho
+        items[i].df, items[i].dg = a, b
+        fo += a + 1; go += b + 1; ho += a + b + 1
+    dF = cm.DeviceArray.from_host(np.concatenate([f for f, _ in pairs]))
+    dG = cm.DeviceArray.from_host(np.concatenate([g for _, g in pairs]))
+    dH = cm.DeviceArray(ho)
+    cm.list_fftmul_dev(items, dF, dG, dH, p)
+    H = dH.download()
+    for i, (a, b) in enumerate(degs):
+        got = H[items[i].h_off:items[i].h_off + a + b + 1]
+        assert (got == orc.poly_mul(pairs[i][0], pairs[i][1], p)).all(), (p, a, b)
 
 
 def test_workload2_full_size(cm, orc):
