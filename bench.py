@@ -591,7 +591,7 @@ def main():
     # and the VALU instructions per launch (SQ_INSTS_VALU): the issue rate,
     # not HBM, bounds these kernels
     traffic, counted = None, {}
-    valu_peak = 1024 * 2.2e9 / 4          # wave-instructions/s: 4 cycles each per SIMD
+    valu_peak = 1024 * 2.32e9 / 4          # wave-instructions/s: 4 cycles each per SIMD
     tpath = os.path.join(ROOT, "profiles", "traffic_2e24.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
