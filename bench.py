@@ -110,6 +110,25 @@ def timed(fn, reps, sync):
     return float(np.median(ts))
 
 
+def graph_replay(cm, fn_on, out, eager, reps):
+    """Capture fn_on(stream) once on a created stream and time its replays
+    (median of reps, after the clock ramp); returns (seconds, every replay
+    equal to the eager output)."""
+    import ctypes as C
+    L = cm.lib()
+    st = L.cumodp_stream_create()
+    try:
+        g = cm.Graph.capture(st, lambda: fn_on(C.c_void_p(st)))
+        sync = lambda: L.cumodp_stream_synchronize(C.c_void_p(st))
+        t = timed(lambda: g.launch(st), reps, sync)
+        sync()
+        same = bool(np.array_equal(out.download(), eager))
+        g.destroy()
+    finally:
+        L.cumodp_stream_destroy(C.c_void_p(st))
+    return t, same
+
+
 def bench_list_multiply(cm, cpu, reps=5):
     """4096 pairs of uneven degree (log-uniform in [32, 16384]) mod 469762049:
     one batched cumodp_list_fftmul_dev call against one cumodp_fftmul_uni_dev
@@ -244,17 +263,20 @@ def bench_tree(cm, cpu, reps=5):
     L = cm.lib()
     nw = L.cumodp_subproduct_tree_words(k)
     dP, dT = cm.DeviceArray.from_host(pts), cm.DeviceArray(nw)
-    run = lambda: cm._check("cumodp_subproduct_tree_dev", L.cumodp_subproduct_tree_dev(
-        dP.addr(), k, p, dT.addr(), None))
+    run_on = lambda st: cm._check("cumodp_subproduct_tree_dev", L.cumodp_subproduct_tree_dev(
+        dP.addr(), k, p, dT.addr(), st))
+    run = lambda: run_on(None)
     run()
     t = timed(run, reps, cm.synchronize)
     tree = dT.download()
+    tg, same_g = graph_replay(cm, run_on, dT, tree, reps)
     root = tree[L.cumodp_subproduct_tree_offset(k, k):]
     r = 123456789
     ok = eval_mod(root, r, p) == _prod_mod([(r - int(x)) % p for x in pts], p)
     dP.free(), dT.free()
     out = {"workload": "subproduct tree, 2^16 points uniform mod 469762049, levels 0..16",
-           "ms": t * 1e3, "tree_words": nw, "check_root_evaluation": bool(ok)}
+           "ms": t * 1e3, "graph_replay_ms": tg * 1e3, "tree_words": nw,
+           "check_root_evaluation": bool(ok), "check_graph_eq_eager": same_g}
     if cpu:
         import oracle
         oracle.lib()
@@ -282,15 +304,18 @@ def bench_eval(cm, cpu, reps=5):
     F = cm.uniform_mod(cm.SplitMix64(616), 1 << k, p)
     L = cm.lib()
     dF, dP, dV = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(pts), cm.DeviceArray(1 << k)
-    run = lambda: cm._check("cumodp_fast_evaluation_dev", L.cumodp_fast_evaluation_dev(
-        dF.addr(), dP.addr(), k, p, dV.addr(), None))
+    run_on = lambda st: cm._check("cumodp_fast_evaluation_dev", L.cumodp_fast_evaluation_dev(
+        dF.addr(), dP.addr(), k, p, dV.addr(), st))
+    run = lambda: run_on(None)
     run()
     t = timed(run, reps, cm.synchronize)
     vals = dV.download()
+    tg, same_g = graph_replay(cm, run_on, dV, vals, reps)
     for d in (dF, dP, dV):
         d.free()
     out = {"workload": "evaluate a degree-(2^16-1) polynomial at 2^16 points mod 469762049",
-           "ms": t * 1e3, "points_per_s": (1 << k) / t}
+           "ms": t * 1e3, "points_per_s": (1 << k) / t, "graph_replay_ms": tg * 1e3,
+           "graph_replay_points_per_s": (1 << k) / tg, "check_graph_eq_eager": same_g}
     if cpu:
         import oracle
         oracle.lib()

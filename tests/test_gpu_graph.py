@@ -75,6 +75,31 @@ def test_graph_subproduct_tree(cm):
         L.cumodp_stream_destroy(C.c_void_p(s))
 
 
+def test_graph_fast_evaluation(cm, orc):
+    """Fast multipoint evaluation (tree, subinverse tree, remainder tree,
+    Horner) captured once and replayed: every replay equals the eager
+    values, which match Horner's rule on a sample of the points."""
+    L = cm.lib()
+    w = cm.workload4_tree(logn=12)
+    k, p = 12, w["p"]
+    F = cm.uniform_mod(cm.SplitMix64(77), 1 << k, p)
+    dF, dP, dV = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(w["points"]), \
+        cm.DeviceArray(1 << k)
+    s = L.cumodp_stream_create()
+    try:
+        run = lambda: cm._check("cumodp_fast_evaluation_dev", L.cumodp_fast_evaluation_dev(
+            dF.addr(), dP.addr(), k, p, dV.addr(), C.c_void_p(s)))
+        g = cm.Graph.capture(s, run)
+        L.cumodp_stream_synchronize(C.c_void_p(s))
+        eager = dV.download()
+        idx = np.array([0, 1, 777, (1 << k) - 1])
+        assert np.array_equal(eager[idx], orc.poly_eval(F, w["points"][idx], p))
+        _replays(cm, s, g, dV, eager, lambda: None)
+        g.destroy()
+    finally:
+        L.cumodp_stream_destroy(C.c_void_p(s))
+
+
 def test_graph_requires_created_stream(cm):
     assert cm.lib().cumodp_graph_begin(None) == cm.CUMODP_ASSUMPTION_ERROR
 
