@@ -81,16 +81,11 @@ static int tblocks(int64_t n) {
     return (int)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
-// Levels whose halves have at most this many non-leading coefficients use
-// the schoolbook kernel (plain.hip).
-constexpr int kTreePlainMaxH = 256;
-
 hipError_t launch_subproduct_tree(const u32 *pts, int k, u32 p, u32 *tree, const Family *fam,
                                   hipStream_t s) {
     const int64_t n = (int64_t)1 << k;
     hipLaunchKernelGGL(k_tree_leaves, dim3(tblocks(n)), dim3(256), 0, s, pts, n, p, tree);
     hipError_t e = hipGetLastError();
-    std::vector<MulDesc> d;
     for (int i = 1; i <= k && e == hipSuccess; ++i) {
         const int h = 1 << (i - 1);
         const int64_t nprod = n >> i;
@@ -101,21 +96,14 @@ hipError_t launch_subproduct_tree(const u32 *pts, int k, u32 p, u32 *tree, const
             e = hipGetLastError();
             continue;
         }
-        // a b of up to 256 coefficients each: schoolbook products (faster than
-        // the transform there); larger: the batched NTT multiply
-        const bool plain = h <= kTreePlainMaxH;
-        d.resize((size_t)nprod);
-        for (int64_t j = 0; j < nprod; ++j) {
-            MulDesc &m = d[(size_t)j];
-            m.f = op + (2 * j) * (h + 1);
-            m.g = m.f + (h + 1);
-            m.h = oc + j * (2 * h + 1);
-            m.lf = m.lg = h;             // the non-leading parts a and b
-            m.lh = 2 * h - 1;
-            m.logn = plain ? 0 : i;      // NTT: 2h points hold the degree 2h-2 product
-        }
-        e = plain ? launch_plain_list(d.data(), nprod, tree, tree, tree, p, s)
-                  : launch_list_mul(d.data(), nprod, tree, tree, tree, fam, s);
+        // the level's products share one shape: a = node 2j and b = node 2j+1
+        // of level i-1 (their non-leading h coefficients, 2(h+1) words apart),
+        // a b (2h-1 words) at node j of level i; no descriptors.  Up to 256
+        // coefficients each the strided multiply takes the schoolbook kernel
+        // (faster than the transform there), above it the NTT multiply on
+        // 2h points
+        e = launch_mul_strided(i, tree + op, 2 * (h + 1), h, tree + op + (h + 1), 2 * (h + 1), h,
+                               tree + oc, 2 * h + 1, 2 * h - 1, (int)nprod, fam, s);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_tree_combine, dim3(tblocks(nprod * (h + 1))), dim3(256), 0, s, tree,
                            op, oc, h, nprod, p);
