@@ -11,7 +11,9 @@
 //     comes from a direct power-series inversion (one thread per node).
 //     Above that, I_(i+1)(j) comes from the two children: their product is
 //     the inverse mod x^(2^i), and one Newton step g <- g (2 - f g) doubles
-//     the precision to 2^(i+1).  That is three batched products per level;
+//     the precision to 2^(i+1).  Only the high half of f g is needed (its
+//     low half is 1), so that is three batched products on 2^(i+1) points
+//     per level and no elementwise pass;
 //   * remainder tree: R_k = F, and R_i(j) = R_(i+1)(j/2) mod M_i(j) by
 //     q = rev(rev(A) I_i(j) mod x^h) and R = (A - q M) mod x^h with h = 2^i.
 //     That is two batched products per level;
@@ -63,16 +65,16 @@ __global__ void k_inv_base(const u32 *__restrict__ M, int64_t nodes, PlainMod pm
     }
 }
 
-// Reversed nodes of every level the subinverse tree needs, in one launch:
-// for level l = first + blockIdx.y (node length h2 + 1, h2 = 2^l),
-// f(j) = rev(M_l(j)) mod x^h2, f_t = M_(h2 - t) for t < h2, stored at
+// Negated reversed nodes of every level the subinverse tree needs, in one
+// launch: for level l = first + blockIdx.y (node length h2 + 1, h2 = 2^l),
+// f(j) = -rev(M_l(j)) mod x^h2, f_t = -M_(h2 - t) for t < h2, stored at
 // out + blockIdx.y n (each level's reversed nodes take n words).
 struct LevelOffsets {
     int64_t off[32];   // word offset of level l in the tree
 };
 
 __global__ void k_rev_nodes(const u32 *__restrict__ tree, LevelOffsets lo, int first, int64_t n,
-                            u32 *__restrict__ out) {
+                            u32 p, u32 *__restrict__ out) {
     const int l = first + (int)blockIdx.y;
     const int h2 = 1 << l;
     const u32 *M = tree + lo.off[l];
@@ -81,17 +83,8 @@ __global__ void k_rev_nodes(const u32 *__restrict__ tree, LevelOffsets lo, int f
          g += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = g >> l;
         const int t = (int)(g & (h2 - 1));
-        f[g] = M[j * (h2 + 1) + (h2 - t)];
-    }
-}
-
-// u = 2 - e (mod p), in place
-__global__ void k_two_minus(u32 *__restrict__ e, int h2, int64_t total, u32 p) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const u32 v = e[g];
-        if ((g & (h2 - 1)) == 0) e[g] = cm_sub(2u % p, v, p);
-        else e[g] = v ? p - v : 0u;
+        const u32 v = M[j * (h2 + 1) + (h2 - t)];
+        f[g] = v ? p - v : 0u;
     }
 }
 
@@ -201,25 +194,34 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         LevelOffsets lo{};
         for (int l = 0; l <= k && l < 32; ++l) lo.off[l] = tree_level_offset(k, l);
         hipLaunchKernelGGL(k_rev_nodes, dim3(eblocks(n), k - 1 - L0), dim3(256), 0, s, tree, lo,
-                           L0 + 1, n, rev);
+                           L0 + 1, n, p, rev);
         e = hipGetLastError();
     }
+    // Newton step from level i to i + 1 (h = 2^i): with f = rev(M_(i+1)(j))
+    // and g0 = its inverse mod x^h, f g0 = 1 + x^h E mod x^(2h), and
+    // g0 (2 - f g0) = g0 - x^h (g0 E) mod x^(2h).  So I_(i+1)(j) keeps g0 as
+    // its low half and takes -(g0 E) mod x^h as its high half: three
+    // products on 2h points per level, no elementwise pass.
     for (int i = L0; i + 1 <= k - 1 && e == hipSuccess; ++i) {
         const int h = 1 << i;
         const int N = (int)(n >> (i + 1));                   // nodes of level i + 1
-        // g0(j) = I_i(2j) I_i(2j+1) mod x^h
-        e = launch_mul_strided(i + 1, inv_at(i), 2 * h, h, inv_at(i) + h, 2 * h, h, T1, h, h, N,
-                               fam, s);
+        u32 *I1 = inv_at(i + 1);                             // node j at I1 + 2h j
+        // g0(j) = I_i(2j) I_i(2j+1) mod x^h (rev is multiplicative), the
+        // low half of I_(i+1)(j)
+        e = launch_mul_strided(i + 1, inv_at(i), 2 * h, h, inv_at(i) + h, 2 * h, h, I1, 2 * h, h,
+                               N, fam, s);
         if (e != hipSuccess) break;
-        // e(j) = f(j) g0(j) mod x^(2h), f(j) = rev(M_(i+1)(j)) mod x^(2h)
-        const u32 *f = rev + (int64_t)(i - L0) * n;
-        e = launch_mul_strided(i + 2, f, 2 * h, 2 * h, T1, h, h, T3, 2 * h, 2 * h, N, fam, s);
+        // (-f) g0: coefficients h .. 2h-1 are -E.  The cyclic product on 2h
+        // points gives them exactly (f g0 has degree < 3h - 1, so the wrapped
+        // terms land below x^(h-1)); short products take the schoolbook
+        // kernel on 4h points instead (same words h .. 2h-1)
+        const u32 *fneg = rev + (int64_t)(i - L0) * n;
+        const int ln2 = h <= list_plain_cutoff() ? i + 2 : i + 1;
+        e = launch_mul_strided(ln2, fneg, 2 * h, 2 * h, I1, 2 * h, h, T3, 2 * h, 2 * h, N, fam, s);
         if (e != hipSuccess) break;
-        hipLaunchKernelGGL(k_two_minus, dim3(eblocks((int64_t)N * 2 * h)), dim3(256), 0, s, T3,
-                           2 * h, (int64_t)N * 2 * h, p);
-        // I_(i+1)(j) = g0(j) (2 - e(j)) mod x^(2h)
-        e = launch_mul_strided(i + 2, T1, h, h, T3, 2 * h, 2 * h, inv_at(i + 1), 2 * h, 2 * h, N,
-                               fam, s);
+        // high half of I_(i+1)(j) = g0 (-E) mod x^h (reads the low halves of
+        // I1, writes the high halves: disjoint words)
+        e = launch_mul_strided(i + 1, I1, 2 * h, h, T3 + h, 2 * h, h, I1 + h, 2 * h, h, N, fam, s);
     }
     if (e != hipSuccess) return e;
     // ---- remainder tree: R_k = F (read in place), then levels k-1 .. L0; the
