@@ -184,6 +184,10 @@ bool set_list_plain_cutoff(int c) {
 
 int list_plain_cutoff() { return g_plain_cutoff; }
 
+// smallest class of 2^14-point products that runs in LDS (one workgroup per
+// CU at a time, so it needs a few per CU to pay)
+constexpr int64_t kLds14MinCount = 512;
+
 // mixed-size tiles of mv = 0, 1, 2 (2048, 4096, 8192 words)
 template <class MD>
 static hipError_t launch_mixed(int mv, dim3 g, const u32 *F, const u32 *G, u32 *H,
@@ -265,6 +269,14 @@ static hipError_t launch_list_mul_ntt(const MulDesc *descs, int64_t count, const
     const int64_t n_small_end = i;
     int64_t n13_end = n_small_end;   // end of the one-product-per-workgroup sizes
     while (n13_end < count && sorted[(size_t)n13_end].logn <= kMaxLdsLog) ++n13_end;
+    // a class of 2^14-point products large enough to fill the chip runs one
+    // product per 1024-thread workgroup too (F and G tiles take 128 KiB of
+    // LDS): no gather/scatter slabs and one launch instead of three
+    {
+        int64_t e14 = n13_end;
+        while (e14 < count && sorted[(size_t)e14].logn == kMaxLdsLog + 1) ++e14;
+        if (e14 - n13_end >= kLds14MinCount) n13_end = e14;
+    }
 
     const size_t desc_bytes = sizeof(MulDesc) * (size_t)count;
     const size_t tile_bytes = sizeof(TileDesc) * tiles.size();

@@ -643,7 +643,7 @@ hipError_t launch_mul_lds_one(int logn, const u32 *F, const u32 *G, u32 *H,
     switch (logn) {
 #define CM_CASE(K) case K: return mul_lds_t<K, true>(F, G, H, d_descs, ndesc, one, fam, s);
         CM_CASE(4) CM_CASE(5) CM_CASE(6) CM_CASE(7) CM_CASE(8) CM_CASE(9) CM_CASE(10)
-        CM_CASE(11) CM_CASE(12) CM_CASE(13)
+        CM_CASE(11) CM_CASE(12) CM_CASE(13) CM_CASE(14)
 #undef CM_CASE
     }
     return hipErrorInvalidValue;

@@ -157,6 +157,32 @@ def test_extreme_residues_multiply(cm, orc, p):
         assert (got == orc.poly_mul(pairs[i][0], pairs[i][1], p)).all(), (p, a, b)
 
 
+@pytest.mark.parametrize("p", [P1, P2])
+def test_list_multiply_lds_2e14_class(cm, orc, p):
+    """600 products of 2^14 points (df + dg + 1 in (8192, 16384], uneven)
+    in one list: a class that large runs one product per 1024-thread
+    workgroup in LDS.  Bit-exact against the one-call-per-product form
+    (four-step at this size) and, on a sample, against the oracle."""
+    rng = np.random.default_rng(14 + p % 7)
+    m = 600
+    df = rng.integers(4096, 8000, m)
+    dg = rng.integers(np.maximum(8192 - df, 300), 16383 - df)   # above the plain cutoff
+    assert ((df + dg + 1 > 8192) & (df + dg + 1 <= 16384)).all()
+    items, nf, ng, nh = cm.make_items(df, dg)
+    F, G = rand_poly(rng, nf, p), rand_poly(rng, ng, p)
+    dF, dG = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(G)
+    dH1, dH2 = cm.DeviceArray(nh), cm.DeviceArray(nh)
+    cm.list_fftmul_dev(items, dF, dG, dH1, p)
+    cm.list_fftmul_dev(items, dF, dG, dH2, p, per_call=True)
+    H1, H2 = dH1.download(), dH2.download()
+    assert np.array_equal(H1, H2)
+    for i in (0, 1, 299, m - 1):
+        it = items[i]
+        f = F[it.f_off:it.f_off + it.df + 1]
+        g = G[it.g_off:it.g_off + it.dg + 1]
+        assert (H1[it.h_off:it.h_off + it.df + it.dg + 1] == orc.ntt_polymul(f, g, p)).all(), i
+
+
 def test_workload2_full_size(cm, orc):
     """The 2^24-point multiply mod 2013265921, bit-exact against the oracle."""
     w = cm.workload2()
