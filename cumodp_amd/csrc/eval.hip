@@ -148,6 +148,66 @@ __global__ void k_horner(const u32 *__restrict__ R, int h, const u32 *__restrict
     }
 }
 
+// The lowest remainder levels and the leaves in one launch.  Workgroup J
+// takes node J of level `pin` (its remainder at A + J 2^pin, blockDim.x =
+// 2^pin threads, pin <= kEvalFuse) and descends its subtree to level L0 in
+// LDS: at level i (h = 2^i) thread t = c h + u computes coefficient u of
+// node c, first qrev_u = sum_(s<=u) A'_(2h-1-s) I_(u-s) (A' the parent's
+// remainder, I = I_i(node)), then q_s = qrev_(h-1-s) and
+// R_u = A'_u - sum_(s<=u) q_s M_(u-s) -- the same division as the level-wide
+// launches, with the products as schoolbook sums in 64-bit accumulators
+// (accum.h).  Then thread t evaluates the leaf holding point J 2^pin + t by
+// Horner's rule, as k_horner does.
+constexpr int kEvalFuse = 8;
+
+__global__ __launch_bounds__(1 << kEvalFuse) void k_rem_descend(
+    const u32 *__restrict__ A, int pin, int L0, const u32 *__restrict__ tree, LevelOffsets lo,
+    const u32 *__restrict__ inv, int64_t n, const u32 *__restrict__ pts, PlainMod pm,
+    u32 *__restrict__ val) {
+    constexpr int BMAX = 1 << kEvalFuse;
+    __shared__ u32 cur[BMAX], qb[BMAX], Il[BMAX], Ml[BMAX + (BMAX >> 4)];
+    const int B = 1 << pin;
+    const int64_t J = blockIdx.x;
+    const int t = threadIdx.x;
+    cur[t] = A[J * B + t];
+    for (int i = pin - 1; i >= L0; --i) {
+        const int h = 1 << i, nodes = B >> i;
+        const int c = t >> i, u = t & (h - 1);
+        // this level's inverses (B words) and node polynomials (nodes (h+1)
+        // words) of the subtree, both contiguous in global memory
+        Il[t] = inv[(int64_t)(i - L0) * n + J * B + t];
+        const u32 *Mg = tree + lo.off[i] + (J * nodes) * (h + 1);
+        for (int x = t; x < nodes * (h + 1); x += B) Ml[x] = Mg[x];
+        __syncthreads();
+        const u32 *par = cur + (c >> 1) * (2 * h);
+        const u32 *I = Il + c * h;
+        u64 acc = 0;
+        for (int s2 = 0; s2 <= u; ++s2) {
+            acc += (u64)par[2 * h - 1 - s2] * I[u - s2];
+            if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
+        }
+        qb[t] = plain_reduce(acc, pm);
+        __syncthreads();
+        const u32 *q = qb + c * h, *M = Ml + c * (h + 1);
+        acc = 0;
+        for (int s2 = 0; s2 <= u; ++s2) {
+            acc += (u64)q[h - 1 - s2] * M[u - s2];
+            if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
+        }
+        const u32 r = cm_sub(par[u], plain_reduce(acc, pm), pm.p);
+        __syncthreads();
+        cur[t] = r;
+    }
+    __syncthreads();
+    const int64_t x = J * B + t;
+    const u32 *r = cur + ((t >> L0) << L0);
+    const u32 zr = cm_mul_tw(pts[x] % pm.p, pm.r2, pm.p, pm.np);   // z 2^32 mod p
+    u32 acc = 0;
+    for (int s2 = (1 << L0) - 1; s2 >= 0; --s2)
+        acc = cm_add(cm_mul_tw(acc, zr, pm.p, pm.np), r[s2], pm.p);
+    val[x] = acc;
+}
+
 int64_t eval_scratch_words(int k) {
     const int64_t n = (int64_t)1 << k;
     const int L0 = k < kEvalBase ? k : kEvalBase;
@@ -227,15 +287,20 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     // ---- remainder tree: R_k = F (read in place), then levels k-1 .. L0; the
     // reversed top halves of a level's dividends come from the previous
     // level's k_div_finish (from F at the first level)
+    // levels below `stop` (from level pin = min(k, kEvalFuse) down) run in
+    // k_rem_descend, one workgroup per node of level pin
+    const int pin = k < kEvalFuse ? k : kEvalFuse;
+    const bool fuse = pin > L0;
+    const int stop = fuse ? pin : L0;
     const u32 *A = F;
     u32 *Rn = R0;
-    if (k - 1 >= L0) {
+    if (k - 1 >= stop) {
         const int h = 1 << (k - 1);
         hipLaunchKernelGGL(k_div_rev_top, dim3(eblocks(n)), dim3(256), 0, s, F, h,
                            n >> (k - 1), T1);
         e = hipGetLastError();
     }
-    for (int i = k - 1; i >= L0 && e == hipSuccess; --i) {
+    for (int i = k - 1; i >= stop && e == hipSuccess; --i) {
         const int h = 1 << i;
         const int N = (int)(n >> i);                         // nodes of level i
         // qrev(j) = revtop(j) I_i(j) mod x^h
@@ -247,12 +312,19 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         e = launch_mul_strided(i + 1, T1, h, h, node(i), h + 1, h, T2, h, h, N, fam, s);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_div_finish, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, A, T2, h,
-                           (int64_t)N, p, Rn, i > L0 ? T1 : nullptr);
+                           (int64_t)N, p, Rn, i > stop ? T1 : nullptr);
         e = hipGetLastError();
         A = Rn;
         Rn = Rn == R0 ? R1 : R0;
     }
     if (e != hipSuccess) return e;
+    if (fuse) {
+        LevelOffsets lo{};
+        for (int l = 0; l <= k && l < 32; ++l) lo.off[l] = tree_level_offset(k, l);
+        hipLaunchKernelGGL(k_rem_descend, dim3((unsigned)(n >> pin)), dim3(1 << pin), 0, s, A,
+                           pin, L0, tree, lo, inv, n, pts, plain_mod(p), values);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_horner, dim3(eblocks(n)), dim3(256), 0, s, A, 1 << L0, pts, n,
                        plain_mod(p), values);
     return hipGetLastError();
