@@ -208,6 +208,91 @@ __global__ __launch_bounds__(1 << kEvalFuse) void k_rem_descend(
     val[x] = acc;
 }
 
+// The lowest subinverse levels in one launch: workgroup J (blockDim.x =
+// 2^pin threads, kEvalBase < pin <= kEvalFuse) takes the nodes of level
+// kEvalBase .. pin under node J of level pin.  The base level comes from the
+// power-series recurrence of k_inv_base (one thread per node); each level
+// above from the Newton step of the level-wide launches, with the three
+// products as schoolbook sums in LDS: thread t = c 2h + v (node c of level
+// i+1, h = 2^i) forms g0_v (v < h), then -E_(v-h) = -sum f_s g0_(v-s)
+// (v >= h, f_s = M_(2h-s)), then the high half sum g0_s (-E)_(v-h-s).
+// Every level is written to the subinverse tree.
+__global__ __launch_bounds__(1 << kEvalFuse) void k_inv_bottom(
+    const u32 *__restrict__ tree, LevelOffsets lo, int pin, PlainMod pm, u32 *__restrict__ inv,
+    int64_t n) {
+    constexpr int BMAX = 1 << kEvalFuse, H0 = 1 << kEvalBase;
+    __shared__ u32 cur[BMAX], nxt[BMAX], tmp[BMAX / 2], Ml[BMAX + BMAX / (2 * H0)];
+    const int B = 1 << pin;
+    const int64_t J = blockIdx.x;
+    const int t = threadIdx.x;
+    const u32 p = pm.p;
+    const int nb = B >> kEvalBase;
+    if (t < nb) {
+        const u32 *m = tree + lo.off[kEvalBase] + (J * nb + t) * (H0 + 1);
+        u32 f[H0], g[H0];
+#pragma unroll
+        for (int s2 = 1; s2 < H0; ++s2) f[s2] = m[H0 - s2];
+        g[0] = 1u;
+#pragma unroll
+        for (int w = 1; w < H0; ++w) {
+            u64 acc = 0;
+#pragma unroll
+            for (int s2 = 1; s2 <= w; ++s2) {
+                acc += (u64)f[s2] * g[w - s2];
+                if ((s2 & 3) == 0) acc = plain_fold(acc, pm.c32);
+            }
+            const u32 r = plain_reduce(acc, pm);
+            g[w] = r ? p - r : 0u;
+        }
+#pragma unroll
+        for (int w = 0; w < H0; ++w) cur[t * H0 + w] = g[w];
+    }
+    __syncthreads();
+    inv[J * B + t] = cur[t];
+    for (int i = kEvalBase; i < pin; ++i) {
+        const int h = 1 << i, nodes = B >> (i + 1);
+        const int c = t >> (i + 1), v = t & (2 * h - 1);
+        const u32 *Mg = tree + lo.off[i + 1] + (J * nodes) * (2 * h + 1);
+        for (int x = t; x < nodes * (2 * h + 1); x += B) Ml[x] = Mg[x];
+        u32 *g0 = nxt + c * (2 * h);
+        if (v < h) {
+            const u32 *ia = cur + (2 * c) * h, *ib = ia + h;
+            u64 acc = 0;
+            for (int s2 = 0; s2 <= v; ++s2) {
+                acc += (u64)ia[s2] * ib[v - s2];
+                if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
+            }
+            g0[v] = plain_reduce(acc, pm);
+        }
+        __syncthreads();
+        if (v >= h) {
+            const u32 *M = Ml + c * (2 * h + 1);
+            u64 acc = 0;
+            for (int s2 = v - h + 1; s2 <= v; ++s2) {
+                acc += (u64)M[2 * h - s2] * g0[v - s2];
+                if (((s2 - v + h - 1) & 3) == 3) acc = plain_fold(acc, pm.c32);
+            }
+            const u32 e = plain_reduce(acc, pm);
+            tmp[c * h + (v - h)] = e ? p - e : 0u;
+        }
+        __syncthreads();
+        if (v >= h) {
+            const int w = v - h;
+            const u32 *me = tmp + c * h;
+            u64 acc = 0;
+            for (int s2 = 0; s2 <= w; ++s2) {
+                acc += (u64)g0[s2] * me[w - s2];
+                if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
+            }
+            g0[v] = plain_reduce(acc, pm);
+        }
+        __syncthreads();
+        cur[t] = nxt[t];
+        inv[(int64_t)(i + 1 - kEvalBase) * n + J * B + t] = nxt[t];
+        __syncthreads();
+    }
+}
+
 int64_t eval_scratch_words(int k) {
     const int64_t n = (int64_t)1 << k;
     const int L0 = k < kEvalBase ? k : kEvalBase;
@@ -229,8 +314,19 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
 
     hipError_t e = launch_subproduct_tree(pts, k, p, tree, fam, s);
     if (e != hipSuccess) return e;
-    // ---- subinverse tree, levels L0 .. k-1
-    {
+    // ---- subinverse tree, levels L0 .. k-1: levels L0 .. pinv in one launch
+    // (k_inv_bottom) when pinv = min(k - 1, kEvalFuse) > L0, else the base
+    // level alone (k_inv_base); Newton steps up from there
+    const int pinv = k - 1 < kEvalFuse ? k - 1 : kEvalFuse;
+    int istart = L0;
+    if (pinv > L0) {
+        LevelOffsets lo{};
+        for (int l = 0; l <= k && l < 32; ++l) lo.off[l] = tree_level_offset(k, l);
+        hipLaunchKernelGGL(k_inv_bottom, dim3((unsigned)(n >> pinv)), dim3(1 << pinv), 0, s,
+                           tree, lo, pinv, plain_mod(p), inv, n);
+        e = hipGetLastError();
+        istart = pinv;
+    } else {
         const int h = 1 << L0;
         static_assert(kEvalBase == 4, "base inverse kernel is instantiated for 16 words");
         if (h == 16)
@@ -262,7 +358,7 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     // g0 (2 - f g0) = g0 - x^h (g0 E) mod x^(2h).  So I_(i+1)(j) keeps g0 as
     // its low half and takes -(g0 E) mod x^h as its high half: three
     // products on 2h points per level, no elementwise pass.
-    for (int i = L0; i + 1 <= k - 1 && e == hipSuccess; ++i) {
+    for (int i = istart; i + 1 <= k - 1 && e == hipSuccess; ++i) {
         const int h = 1 << i;
         const int N = (int)(n >> (i + 1));                   // nodes of level i + 1
         u32 *I1 = inv_at(i + 1);                             // node j at I1 + 2h j
