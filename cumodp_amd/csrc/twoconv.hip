@@ -164,15 +164,114 @@ __global__ void k_tc_split(const u64 *__restrict__ src, const TcDesc *__restrict
     }
 }
 
+// k_tc_split for one-word coefficients fused with the forward x transform:
+// thread i forms the K = 2^LK2 / 2 limbs of coefficient i (upper K2 - K
+// limbs zero), runs the natural-order length-K2 DFT on them in registers
+// (DIF stages with the x root's level table lvx, zero upper half) and
+// writes out[(k R + r) Ly + i] = DFT_k for every k < K2.  Positions
+// i >= n are zero rows.
+template <int LK2>
+__global__ void k_tc_split_x(const u64 *__restrict__ src, const TcDesc *__restrict__ d,
+                             TcDesc one, int which, int M, const u32 *__restrict__ lvx, Mod md,
+                             u32 *__restrict__ out, int64_t Ly, int R) {
+    constexpr int K2 = 1 << LK2, K = K2 / 2;
+    const int r = blockIdx.y;
+    const TcDesc dd = d ? d[r] : one;
+    const u64 *a = src + (which ? dd.b_off : dd.a_off);
+    const int64_t n = which ? dd.nb : dd.na;
+    u32 *o = out + (int64_t)r * Ly;
+    const int64_t js = (int64_t)R * Ly;
+    const u64 mask = (1ull << M) - 1;
+    const u32 p = md.p;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Ly;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        u32 v[1][K2];
+#pragma unroll
+        for (int j = 0; j < K2; ++j) v[0][j] = 0u;
+        if (i < n) {
+            const u64 x = a[i];
+            const bool neg = (int64_t)x < 0;
+            const u64 mag = neg ? ~x + 1 : x;            // |x| (2^63 for the minimum)
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const int sh = j * M;
+                const u32 limb = sh < 64 ? (u32)((mag >> sh) & mask) : 0u;
+                v[0][j] = neg ? (limb ? p - limb : 0u) : limb;
+            }
+        }
+        cm_dif_stages<LK2, LK2, 0, 0, 1, true>(v, 0, lvx, md);
+#pragma unroll
+        for (int m = 0; m < K2; ++m) o[(int64_t)cm_brev((u32)m, LK2) * js + i] = v[0][m];
+    }
+}
+
+// Garner constants of the CRT over the first two or three primes
+struct TcCrt {
+    u32 p0, p1, p2, inv01, inv01q, inv012, inv012q, p0m2, p0m2q;
+};
+
+// Adds z 2^sh to the wc-word two's-complement accumulator o, where z is the
+// signed CRT value (in (-P/2, P/2]) of the residues r0, r1 (, r2).
+template <int NP>
+__device__ __forceinline__ void tc_crt_add(u64 *o, int wc, int sh, u32 r0, u32 r1, u32 r2,
+                                           const TcCrt &k) {
+    // Garner: t1 = (r1 - r0) p0^-1 mod p1;  z = r0 + p0 t1 (+ p0 p1 t2)
+    const u32 d1 = cm_sub(r1, r0 % k.p1, k.p1);
+    const u32 t1 = cm_shoup(d1, k.inv01, k.inv01q, k.p1);
+    unsigned __int128 z = (unsigned __int128)r0 + (unsigned __int128)k.p0 * t1;
+    unsigned __int128 P = (unsigned __int128)k.p0 * k.p1;
+    if (NP == 3) {
+        // t2 = (r2 - (r0 + p0 t1)) (p0 p1)^-1 mod p2
+        const u32 zm =
+            (u32)(((u64)(r0 % k.p2) + cm_shoup(t1 % k.p2, k.p0m2, k.p0m2q, k.p2)) % k.p2);
+        const u32 d2 = cm_sub(r2, zm, k.p2);
+        const u32 t2 = cm_shoup(d2, k.inv012, k.inv012q, k.p2);
+        z += P * t2;
+        P *= k.p2;
+    }
+    // signed value in (-P/2, P/2]
+    __int128 sz = (__int128)z;
+    if (z > P / 2) sz -= (__int128)P;
+    if (sz == 0) return;
+    // add sz << sh into the wc-word accumulator
+    const int w0 = sh >> 6, b0 = sh & 63;
+    const bool neg = sz < 0;
+    unsigned __int128 mag = (unsigned __int128)(neg ? -sz : sz);
+    u64 part[3];
+    part[0] = (u64)(mag << b0);
+    part[1] = (u64)((mag << b0) >> 64);
+    part[2] = b0 ? (u64)(mag >> (128 - b0)) : 0;
+    if (!neg) {
+        u64 c = 0;
+        for (int q = w0; q < wc; ++q) {
+            const u64 s = (q - w0 < 3) ? part[q - w0] : 0;
+            const u64 t = o[q] + s;
+            const u64 c1 = t < s;
+            const u64 t2 = t + c;
+            c = c1 | (t2 < c);
+            o[q] = t2;
+            if (q - w0 >= 2 && !c) break;
+        }
+    } else {
+        u64 b = 0;
+        for (int q = w0; q < wc; ++q) {
+            const u64 s = (q - w0 < 3) ? part[q - w0] : 0;
+            const u64 v = o[q];
+            const u64 t = v - s - b;
+            b = (v < s) || (v - s < b);
+            o[q] = t;
+            if (q - w0 >= 2 && !b) break;
+        }
+    }
+}
+
 // CRT and recombination for a batch: output coefficient y of item r is
 // sum_j z_j 2^(M j), where z_j (j < 2K-1) is the signed CRT value of the
 // residues C_q[(j R + r) Ly + y].  grid.y: items.
 template <int NP>
 __global__ void k_tc_crt(const u32 *__restrict__ C0, const u32 *__restrict__ C1,
                          const u32 *__restrict__ C2, const TcDesc *__restrict__ d, TcDesc one,
-                         int K2, int M, int64_t Ly, int R, u64 *__restrict__ out, u32 p0,
-                         u32 p1, u32 p2, u32 inv01, u32 inv01q, u32 inv012, u32 inv012q,
-                         u32 p0m2, u32 p0m2q) {
+                         int K2, int M, int64_t Ly, int R, u64 *__restrict__ out, TcCrt k) {
     const int r = blockIdx.y;
     const TcDesc dd = d ? d[r] : one;
     const int64_t ny = dd.na + dd.nb - 1;
@@ -184,56 +283,53 @@ __global__ void k_tc_crt(const u32 *__restrict__ C0, const u32 *__restrict__ C1,
         for (int q = 0; q < wc; ++q) o[q] = 0;
         for (int j = 0; j < K2 - 1; ++j) {
             const int64_t at = j * js + base + y;
-            const u32 r0 = C0[at], r1 = C1[at];
-            // Garner: t1 = (r1 - r0) p0^-1 mod p1;  z = r0 + p0 t1 (+ p0 p1 t2)
-            const u32 d1 = cm_sub(r1, r0 % p1, p1);
-            const u32 t1 = cm_shoup(d1, inv01, inv01q, p1);
-            unsigned __int128 z = (unsigned __int128)r0 + (unsigned __int128)p0 * t1;
-            unsigned __int128 P = (unsigned __int128)p0 * p1;
-            if (NP == 3) {
-                const u32 r2 = C2[at];
-                // t2 = (r2 - (r0 + p0 t1)) (p0 p1)^-1 mod p2
-                const u32 zm = (u32)(((u64)(r0 % p2) + cm_shoup(t1 % p2, p0m2, p0m2q, p2)) % p2);
-                const u32 d2 = cm_sub(r2, zm, p2);
-                const u32 t2 = cm_shoup(d2, inv012, inv012q, p2);
-                z += P * t2;
-                P *= p2;
-            }
-            // signed value in (-P/2, P/2]
-            __int128 sz = (__int128)z;
-            if (z > P / 2) sz -= (__int128)P;
-            if (sz == 0) continue;
-            // add sz << (M j) into the wc-word accumulator
-            const int sh = M * j, w0 = sh >> 6, b0 = sh & 63;
-            const bool neg = sz < 0;
-            unsigned __int128 mag = (unsigned __int128)(neg ? -sz : sz);
-            u64 part[3];
-            part[0] = (u64)(mag << b0);
-            part[1] = (u64)((mag << b0) >> 64);
-            part[2] = b0 ? (u64)(mag >> (128 - b0)) : 0;
-            if (!neg) {
-                u64 c = 0;
-                for (int q = w0; q < wc; ++q) {
-                    const u64 s = (q - w0 < 3) ? part[q - w0] : 0;
-                    const u64 t = o[q] + s;
-                    const u64 c1 = t < s;
-                    const u64 t2 = t + c;
-                    c = c1 | (t2 < c);
-                    o[q] = t2;
-                    if (q - w0 >= 2 && !c) break;
-                }
-            } else {
-                u64 b = 0;
-                for (int q = w0; q < wc; ++q) {
-                    const u64 s = (q - w0 < 3) ? part[q - w0] : 0;
-                    const u64 v = o[q];
-                    const u64 t = v - s - b;
-                    b = (v < s) || (v - s < b);
-                    o[q] = t;
-                    if (q - w0 >= 2 && !b) break;
-                }
-            }
+            tc_crt_add<NP>(o, wc, M * j, C0[at], C1[at], NP == 3 ? C2[at] : 0u, k);
         }
+    }
+}
+
+// Inverse x transforms of the residues of each prime (natural-order length-K2
+// DFT with the inverse root's level table, times 1/K2) fused with the CRT
+// and recombination: thread y loads C_q[(k R + r) Ly + y] for k < K2 and
+// prime q, transforms them in registers, then adds z_j 2^(M j) for
+// j < K2 - 1 as k_tc_crt does.
+struct TcInvX {
+    const u32 *lv[3];   // inverse x root's level tables, per prime
+    u32 np[3];          // -p^-1 mod 2^32, per prime
+    u32 ki[3];          // Montgomery form of 1 / K2, per prime
+};
+
+template <int NP, int LK2>
+__global__ void k_tc_crt_x(const u32 *__restrict__ C0, const u32 *__restrict__ C1,
+                           const u32 *__restrict__ C2, const TcDesc *__restrict__ d, TcDesc one,
+                           int M, int64_t Ly, int R, u64 *__restrict__ out, TcCrt k, TcInvX ix) {
+    constexpr int K2 = 1 << LK2;
+    const int r = blockIdx.y;
+    const TcDesc dd = d ? d[r] : one;
+    const int64_t ny = dd.na + dd.nb - 1;
+    const int wc = dd.wc;
+    const int64_t js = (int64_t)R * Ly, base = (int64_t)r * Ly;
+    const u32 ps[3] = {k.p0, k.p1, k.p2};
+    const u32 *Cs[3] = {C0, C1, C2};
+    for (int64_t y = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; y < ny;
+         y += (int64_t)gridDim.x * blockDim.x) {
+        u32 res[NP][K2];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+            u32 v[1][K2];
+#pragma unroll
+            for (int j = 0; j < K2; ++j) v[0][j] = Cs[q][j * js + base + y];
+            const Mod md{ps[q], ix.np[q]};
+            cm_dif_stages<LK2, LK2, 0, 0, 1>(v, 0, ix.lv[q], md);
+#pragma unroll
+            for (int j = 0; j < K2; ++j)
+                res[q][j] = cm_mul_tw(v[0][cm_brev((u32)j, LK2)], ix.ki[q], md.p, md.np);
+        }
+        u64 *o = out + dd.c_off + y * wc;
+        for (int q = 0; q < wc; ++q) o[q] = 0;
+#pragma unroll
+        for (int j = 0; j < K2 - 1; ++j)
+            tc_crt_add<NP>(o, wc, M * j, res[0][j], res[1][j], NP == 3 ? res[2][j] : 0u, k);
     }
 }
 
@@ -282,61 +378,111 @@ int tc_bit_width(const u64 *a, int64_t n, int w) {
     return best;
 }
 
+// forward split + x transform of one operand (k_tc_split_x), LK2 = 1 .. 3
+static void tc_split_x(int lk2, dim3 g, const u64 *src, const TcDesc *d, const TcDesc &one,
+                       int which, int M, const u32 *lvx, Mod md, u32 *out, int64_t Ly, int R,
+                       hipStream_t s) {
+#define CM_SX(L)                                                                             \
+    case L:                                                                                  \
+        hipLaunchKernelGGL(k_tc_split_x<L>, g, dim3(256), 0, s, src, d, one, which, M, lvx, \
+                           md, out, Ly, R);                                                  \
+        break;
+    switch (lk2) { CM_SX(1) CM_SX(2) CM_SX(3) default: break; }
+#undef CM_SX
+}
+
 // The products of one class (same plan and y size) over R items: per prime,
 // A and B are 2K x R x Ly (limb row j of item r at (j R + r) Ly), so the x
 // transforms are column transforms over R Ly columns and the y products are
-// 2K R rows of Ly words.  d: device descriptors, or null with `one`.
+// 2K R rows of Ly words.  d: device descriptors, or null with `one`.  w1:
+// every coefficient of the class is one word; then, for 2K <= 8, the x
+// transforms run inside the split and CRT kernels (k_tc_split_x,
+// k_tc_crt_x) instead of as three column passes per prime.
 static hipError_t tc_run_class(const TcPlan &pl, const u64 *A, const u64 *B, u64 *C,
-                               const TcDesc *d, const TcDesc &one, int R, int64_t ymax,
+                               const TcDesc *d, const TcDesc &one, int R, int64_t ymax, bool w1,
                                u32 *ws, hipStream_t s) {
     const int K2 = 1 << pl.lk2;
     const int64_t Ly = (int64_t)1 << pl.ey;
     const size_t per = (size_t)2 * K2 * R * Ly;   // A and B rows of one prime
     const int bx = blocks_for(Ly);
+    const bool fused = w1 && pl.lk2 >= 1 && pl.lk2 <= 3;   // 2K = 16 would spill in the CRT
     u32 *Cr[3] = {nullptr, nullptr, nullptr};
+    TcInvX ix{};
     for (int q = 0; q < pl.np; ++q) {
         const u32 p = kTcPrimes[q];
         const Family *fam = family_default(p);
         if (!fam) return hipErrorOutOfMemory;
         u32 *Aq = ws + per * q, *Bq = Aq + (size_t)K2 * R * Ly;
         Cr[q] = Aq;
-        {
-            ProfScope ps(kProfTcSplit, s);
-            hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, A, d, one, 0, pl.K, K2,
-                               pl.M, p, Aq, Ly, R);
-            hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, B, d, one, 1, pl.K, K2,
-                               pl.M, p, Bq, Ly, R);
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
         // x: length-2K DFT down the columns (cyclic points even, negacyclic odd)
         const u32 wx = cm_host_pow(fam->wN, (u64)1 << (fam->K - pl.lk2), p);
         const Family *fx = family_for_root(p, wx, pl.lk2);
         const Family *fxi = family_for_root(p, cm_host_inv(wx, p), pl.lk2);
         if (!fx || !fxi) return hipErrorOutOfMemory;
-        e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, 1u, fx, s);
-        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Bq, (int64_t)R * Ly, 1u, fx, s);
+        const u32 ki = cm_host_inv((u32)K2, p);
+        hipError_t e;
+        if (fused) {
+            ProfScope ps(kProfTcSplit, s);
+            const Mod md{p, fam->np};
+            tc_split_x(pl.lk2, dim3(bx, R), A, d, one, 0, pl.M, fx->lv, md, Aq, Ly, R, s);
+            tc_split_x(pl.lk2, dim3(bx, R), B, d, one, 1, pl.M, fx->lv, md, Bq, Ly, R, s);
+            e = hipGetLastError();
+            ix.lv[q] = fxi->lv;
+            ix.np[q] = fam->np;
+            ix.ki[q] = cm_host_mont(ki, p);
+        } else {
+            {
+                ProfScope ps(kProfTcSplit, s);
+                hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, A, d, one, 0, pl.K,
+                                   K2, pl.M, p, Aq, Ly, R);
+                hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, B, d, one, 1, pl.K,
+                                   K2, pl.M, p, Bq, Ly, R);
+            }
+            e = hipGetLastError();
+            if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, 1u, fx, s);
+            if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Bq, (int64_t)R * Ly, 1u, fx, s);
+        }
         // y: 2K R independent cyclic products of length Ly (= linear: ny <= Ly)
         if (e == hipSuccess) e = launch_rows_mul(pl.ey, Aq, Bq, K2 * R, fam, s);
         // inverse x transform with the 1/(2K) scaling
-        const u32 ki = cm_host_inv((u32)K2, p);
-        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, ki, fxi, s);
+        if (e == hipSuccess && !fused)
+            e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, ki, fxi, s);
         if (e != hipSuccess) return e;
     }
     const u32 p0 = kTcPrimes[0], p1 = kTcPrimes[1], p2 = kTcPrimes[2];
     const u32 inv01 = cm_host_inv(p0 % p1, p1);
     const u32 p0m2 = p0 % p2;
     const u32 inv012 = cm_host_inv(cm_host_mul(p0 % p2, p1 % p2, p2), p2);
+    const TcCrt kc{p0, p1, p2, inv01, cm_host_shoup_q(inv01, p1), inv012,
+                   cm_host_shoup_q(inv012, p2), p0m2, cm_host_shoup_q(p0m2, p2)};
     ProfScope ps(kProfTcCrt, s);
     const dim3 g(blocks_for(ymax), R);
-    if (pl.np == 2)
-        hipLaunchKernelGGL((k_tc_crt<2>), g, dim3(256), 0, s, Cr[0], Cr[1], Cr[1], d, one, K2,
-                           pl.M, Ly, R, C, p0, p1, p2, inv01, cm_host_shoup_q(inv01, p1), inv012,
-                           cm_host_shoup_q(inv012, p2), p0m2, cm_host_shoup_q(p0m2, p2));
-    else
-        hipLaunchKernelGGL((k_tc_crt<3>), g, dim3(256), 0, s, Cr[0], Cr[1], Cr[2], d, one, K2,
-                           pl.M, Ly, R, C, p0, p1, p2, inv01, cm_host_shoup_q(inv01, p1), inv012,
-                           cm_host_shoup_q(inv012, p2), p0m2, cm_host_shoup_q(p0m2, p2));
+    const u32 *C2 = pl.np == 3 ? Cr[2] : Cr[1];
+    if (fused) {
+#define CM_CX(NP, L)                                                                        \
+    hipLaunchKernelGGL((k_tc_crt_x<NP, L>), g, dim3(256), 0, s, Cr[0], Cr[1], C2, d, one, pl.M, \
+                       Ly, R, C, kc, ix)
+        if (pl.np == 2) {
+            switch (pl.lk2) {
+            case 1: CM_CX(2, 1); break;
+            case 2: CM_CX(2, 2); break;
+            default: CM_CX(2, 3); break;
+            }
+        } else {
+            switch (pl.lk2) {
+            case 1: CM_CX(3, 1); break;
+            case 2: CM_CX(3, 2); break;
+            default: CM_CX(3, 3); break;
+            }
+        }
+#undef CM_CX
+    } else if (pl.np == 2) {
+        hipLaunchKernelGGL((k_tc_crt<2>), g, dim3(256), 0, s, Cr[0], Cr[1], C2, d, one, K2, pl.M,
+                           Ly, R, C, kc);
+    } else {
+        hipLaunchKernelGGL((k_tc_crt<3>), g, dim3(256), 0, s, Cr[0], Cr[1], C2, d, one, K2, pl.M,
+                           Ly, R, C, kc);
+    }
     return hipGetLastError();
 }
 
@@ -354,7 +500,7 @@ hipError_t launch_two_convolution(const u64 *a, int64_t na, int wa, const u64 *b
     u32 *ws = (u32 *)workspace(10, sizeof(u32) * per * pl.np, s);
     if (!ws) return hipErrorOutOfMemory;
     TcDesc one{0, 0, 0, na, nb, wa, wb, wc, 0};
-    return tc_run_class(pl, a, b, c, nullptr, one, 1, na + nb - 1, ws, s);
+    return tc_run_class(pl, a, b, c, nullptr, one, 1, na + nb - 1, wa == 1 && wb == 1, ws, s);
 }
 
 // Batch of products: items are grouped by class plan (tc_class_plan: one plan
@@ -457,12 +603,14 @@ hipError_t launch_two_convolution_list(const TcItem *items, int64_t count, const
             u32 *ws = (u32 *)workspace(10, item_bytes * (size_t)R, cs);
             if (!ws) return hipErrorOutOfMemory;
             int64_t ymax = 1;
+            bool w1 = true;
             for (int r = 0; r < R; ++r) {
                 const TcItem &it = items[c.idx[(size_t)(k0 + r)]];
                 ymax = std::max<int64_t>(ymax, it.na + it.nb - 1);
+                w1 = w1 && it.wa == 1 && it.wb == 1;
             }
             TcDesc none{};
-            e = tc_run_class(c.pl, A, B, C, dd + first + k0, none, R, ymax, ws, cs);
+            e = tc_run_class(c.pl, A, B, C, dd + first + k0, none, R, ymax, w1, ws, cs);
             if (e != hipSuccess) return e;
         }
         first += (size_t)n;
