@@ -68,6 +68,26 @@ def test_signed_random(cm, orc, bits):
         assert (got == orc.zmul_i64(a, b)).all(), (bits, na, nb)
 
 
+@pytest.mark.parametrize("bits", [20, 40, 62])
+def test_one_word_fused_equals_two_word(cm, orc, bits):
+    """One-word coefficients take the fused path (x transforms inside the
+    split and CRT kernels); the same values given as two-word
+    two's-complement coefficients take the column-pass path.  Both equal the
+    oracle's exact product."""
+    rng = np.random.default_rng(100 + bits)
+    lim = (1 << bits) - 1
+    for na, nb in [(1, 1), (7, 300), (1000, 999), (4096, 2500)]:
+        a = rng.integers(-lim, lim, na, dtype=np.int64)
+        b = rng.integers(-lim, lim, nb, dtype=np.int64)
+        one = cm.two_convolution_poly_mul(a, b, 3)
+        a2 = np.stack([a.view(np.uint64), np.where(a < 0, np.uint64(2**64 - 1), np.uint64(0))], 1)
+        b2 = np.stack([b.view(np.uint64), np.where(b < 0, np.uint64(2**64 - 1), np.uint64(0))], 1)
+        two = cm.two_convolution_poly_mul(a2, b2, 3)
+        assert np.array_equal(one, two), (bits, na, nb)
+        want = orc.zmul_i64(a, b)
+        assert (one == want).all(), (bits, na, nb)
+
+
 def test_extreme_and_zero(cm, orc):
     top = 2 ** 63 - 1
     a = np.array([-top, top, 0, -1, 1, -top], np.int64)
