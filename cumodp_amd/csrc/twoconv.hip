@@ -164,25 +164,33 @@ __global__ void k_tc_split(const u64 *__restrict__ src, const TcDesc *__restrict
     }
 }
 
-// k_tc_split for one-word coefficients fused with the forward x transform:
-// thread i forms the K = 2^LK2 / 2 limbs of coefficient i (upper K2 - K
-// limbs zero), runs the natural-order length-K2 DFT on them in registers
-// (DIF stages with the x root's level table lvx, zero upper half) and
-// writes out[(k R + r) Ly + i] = DFT_k for every k < K2.  Positions
-// i >= n are zero rows.
+// k_tc_split for one-word coefficients fused with the forward x transform,
+// for both operands and every prime in one launch (grid.z = 2 q + which):
+// thread i forms the K = 2^LK2 / 2 limbs of coefficient i of operand
+// `which` (upper K2 - K limbs zero), runs the natural-order length-K2 DFT
+// on them in registers (DIF stages with prime q's x-root level table, zero
+// upper half) and writes out[q][which][(k R + r) Ly + i] = DFT_k for every
+// k < K2.  Positions i >= n are zero rows.
+struct TcFwdX {
+    const u32 *lv[3];   // x root's level tables, per prime
+    u32 p[3], np[3];
+    u32 *out[3][2];     // A and B arrays of each prime
+};
+
 template <int LK2>
-__global__ void k_tc_split_x(const u64 *__restrict__ src, const TcDesc *__restrict__ d,
-                             TcDesc one, int which, int M, const u32 *__restrict__ lvx, Mod md,
-                             u32 *__restrict__ out, int64_t Ly, int R) {
+__global__ void k_tc_split_x(const u64 *__restrict__ A, const u64 *__restrict__ B,
+                             const TcDesc *__restrict__ d, TcDesc one, int M, TcFwdX fx,
+                             int64_t Ly, int R) {
     constexpr int K2 = 1 << LK2, K = K2 / 2;
-    const int r = blockIdx.y;
+    const int r = blockIdx.y, q = blockIdx.z >> 1, which = blockIdx.z & 1;
     const TcDesc dd = d ? d[r] : one;
-    const u64 *a = src + (which ? dd.b_off : dd.a_off);
+    const u64 *a = which ? B + dd.b_off : A + dd.a_off;
     const int64_t n = which ? dd.nb : dd.na;
-    u32 *o = out + (int64_t)r * Ly;
+    u32 *o = fx.out[q][which] + (int64_t)r * Ly;
     const int64_t js = (int64_t)R * Ly;
     const u64 mask = (1ull << M) - 1;
-    const u32 p = md.p;
+    const Mod md{fx.p[q], fx.np[q]};
+    const u32 *lvx = fx.lv[q];
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Ly;
          i += (int64_t)gridDim.x * blockDim.x) {
         u32 v[1][K2];
@@ -196,7 +204,7 @@ __global__ void k_tc_split_x(const u64 *__restrict__ src, const TcDesc *__restri
             for (int j = 0; j < K; ++j) {
                 const int sh = j * M;
                 const u32 limb = sh < 64 ? (u32)((mag >> sh) & mask) : 0u;
-                v[0][j] = neg ? (limb ? p - limb : 0u) : limb;
+                v[0][j] = neg ? (limb ? md.p - limb : 0u) : limb;
             }
         }
         cm_dif_stages<LK2, LK2, 0, 0, 1, true>(v, 0, lvx, md);
@@ -378,19 +386,6 @@ int tc_bit_width(const u64 *a, int64_t n, int w) {
     return best;
 }
 
-// forward split + x transform of one operand (k_tc_split_x), LK2 = 1 .. 3
-static void tc_split_x(int lk2, dim3 g, const u64 *src, const TcDesc *d, const TcDesc &one,
-                       int which, int M, const u32 *lvx, Mod md, u32 *out, int64_t Ly, int R,
-                       hipStream_t s) {
-#define CM_SX(L)                                                                             \
-    case L:                                                                                  \
-        hipLaunchKernelGGL(k_tc_split_x<L>, g, dim3(256), 0, s, src, d, one, which, M, lvx, \
-                           md, out, Ly, R);                                                  \
-        break;
-    switch (lk2) { CM_SX(1) CM_SX(2) CM_SX(3) default: break; }
-#undef CM_SX
-}
-
 // The products of one class (same plan and y size) over R items: per prime,
 // A and B are 2K x R x Ly (limb row j of item r at (j R + r) Ly), so the x
 // transforms are column transforms over R Ly columns and the y products are
@@ -408,10 +403,13 @@ static hipError_t tc_run_class(const TcPlan &pl, const u64 *A, const u64 *B, u64
     const bool fused = w1 && pl.lk2 >= 1 && pl.lk2 <= 3;   // 2K = 16 would spill in the CRT
     u32 *Cr[3] = {nullptr, nullptr, nullptr};
     TcInvX ix{};
+    TcFwdX fw{};
+    const Family *fams[3] = {nullptr, nullptr, nullptr};
     for (int q = 0; q < pl.np; ++q) {
         const u32 p = kTcPrimes[q];
         const Family *fam = family_default(p);
         if (!fam) return hipErrorOutOfMemory;
+        fams[q] = fam;
         u32 *Aq = ws + per * q, *Bq = Aq + (size_t)K2 * R * Ly;
         Cr[q] = Aq;
         // x: length-2K DFT down the columns (cyclic points even, negacyclic odd)
@@ -420,33 +418,50 @@ static hipError_t tc_run_class(const TcPlan &pl, const u64 *A, const u64 *B, u64
         const Family *fxi = family_for_root(p, cm_host_inv(wx, p), pl.lk2);
         if (!fx || !fxi) return hipErrorOutOfMemory;
         const u32 ki = cm_host_inv((u32)K2, p);
-        hipError_t e;
         if (fused) {
-            ProfScope ps(kProfTcSplit, s);
-            const Mod md{p, fam->np};
-            tc_split_x(pl.lk2, dim3(bx, R), A, d, one, 0, pl.M, fx->lv, md, Aq, Ly, R, s);
-            tc_split_x(pl.lk2, dim3(bx, R), B, d, one, 1, pl.M, fx->lv, md, Bq, Ly, R, s);
-            e = hipGetLastError();
+            fw.lv[q] = fx->lv;
+            fw.p[q] = p;
+            fw.np[q] = fam->np;
+            fw.out[q][0] = Aq;
+            fw.out[q][1] = Bq;
             ix.lv[q] = fxi->lv;
             ix.np[q] = fam->np;
             ix.ki[q] = cm_host_mont(ki, p);
-        } else {
-            {
-                ProfScope ps(kProfTcSplit, s);
-                hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, A, d, one, 0, pl.K,
-                                   K2, pl.M, p, Aq, Ly, R);
-                hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, B, d, one, 1, pl.K,
-                                   K2, pl.M, p, Bq, Ly, R);
-            }
-            e = hipGetLastError();
-            if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, 1u, fx, s);
-            if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Bq, (int64_t)R * Ly, 1u, fx, s);
+            continue;
         }
+        hipError_t e;
+        {
+            ProfScope ps(kProfTcSplit, s);
+            hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, A, d, one, 0, pl.K, K2,
+                               pl.M, p, Aq, Ly, R);
+            hipLaunchKernelGGL(k_tc_split, dim3(bx, R), dim3(256), 0, s, B, d, one, 1, pl.K, K2,
+                               pl.M, p, Bq, Ly, R);
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, 1u, fx, s);
+        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Bq, (int64_t)R * Ly, 1u, fx, s);
         // y: 2K R independent cyclic products of length Ly (= linear: ny <= Ly)
         if (e == hipSuccess) e = launch_rows_mul(pl.ey, Aq, Bq, K2 * R, fam, s);
         // inverse x transform with the 1/(2K) scaling
-        if (e == hipSuccess && !fused)
-            e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, ki, fxi, s);
+        if (e == hipSuccess) e = launch_col_ntt(pl.lk2, Aq, (int64_t)R * Ly, ki, fxi, s);
+        if (e != hipSuccess) return e;
+    }
+    if (fused) {
+        // both operands of every prime split and transformed in one launch,
+        // then the y products prime by prime
+        {
+            ProfScope ps(kProfTcSplit, s);
+            const dim3 g(bx, R, 2 * pl.np);
+            if (pl.lk2 == 1)
+                hipLaunchKernelGGL(k_tc_split_x<1>, g, dim3(256), 0, s, A, B, d, one, pl.M, fw, Ly, R);
+            else if (pl.lk2 == 2)
+                hipLaunchKernelGGL(k_tc_split_x<2>, g, dim3(256), 0, s, A, B, d, one, pl.M, fw, Ly, R);
+            else
+                hipLaunchKernelGGL(k_tc_split_x<3>, g, dim3(256), 0, s, A, B, d, one, pl.M, fw, Ly, R);
+        }
+        hipError_t e = hipGetLastError();
+        for (int q = 0; q < pl.np && e == hipSuccess; ++q)
+            e = launch_rows_mul(pl.ey, fw.out[q][0], fw.out[q][1], K2 * R, fams[q], s);
         if (e != hipSuccess) return e;
     }
     const u32 p0 = kTcPrimes[0], p1 = kTcPrimes[1], p2 = kTcPrimes[2];
