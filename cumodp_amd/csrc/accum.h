@@ -31,6 +31,27 @@ __device__ __forceinline__ u32 plain_reduce(u64 acc, const PlainMod &m) {
 }
 
 
+// sum_(k < cnt) x[k DX] y[k DY] mod p for canonical residues (DX, DY = +-1),
+// four products per fold.  The four pairs of a group are loaded before any
+// is multiplied, so their LDS or cache latencies overlap.
+template <int DX, int DY>
+__device__ __forceinline__ u32 plain_dot(const u32 *x, const u32 *y, int cnt,
+                                         const PlainMod &m) {
+    u64 acc = 0;
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        const u32 x0 = x[k * DX], x1 = x[(k + 1) * DX], x2 = x[(k + 2) * DX], x3 = x[(k + 3) * DX];
+        const u32 y0 = y[k * DY], y1 = y[(k + 1) * DY], y2 = y[(k + 2) * DY], y3 = y[(k + 3) * DY];
+        acc += (u64)x0 * y0;
+        acc += (u64)x1 * y1;
+        acc += (u64)x2 * y2;
+        acc += (u64)x3 * y3;
+        acc = plain_fold(acc, m.c32);
+    }
+    for (; k < cnt; ++k) acc += (u64)x[k * DX] * y[k * DY];
+    return plain_reduce(acc, m);
+}
+
 // host: the constants for modulus p (odd, 3 <= p < 2^31)
 static inline PlainMod plain_mod(u32 p) {
     PlainMod m;

@@ -181,20 +181,10 @@ __global__ __launch_bounds__(1 << kEvalFuse) void k_rem_descend(
         __syncthreads();
         const u32 *par = cur + (c >> 1) * (2 * h);
         const u32 *I = Il + c * h;
-        u64 acc = 0;
-        for (int s2 = 0; s2 <= u; ++s2) {
-            acc += (u64)par[2 * h - 1 - s2] * I[u - s2];
-            if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
-        }
-        qb[t] = plain_reduce(acc, pm);
+        qb[t] = plain_dot<-1, -1>(par + 2 * h - 1, I + u, u + 1, pm);
         __syncthreads();
         const u32 *q = qb + c * h, *M = Ml + c * (h + 1);
-        acc = 0;
-        for (int s2 = 0; s2 <= u; ++s2) {
-            acc += (u64)q[h - 1 - s2] * M[u - s2];
-            if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
-        }
-        const u32 r = cm_sub(par[u], plain_reduce(acc, pm), pm.p);
+        const u32 r = cm_sub(par[u], plain_dot<-1, -1>(q + h - 1, M + u, u + 1, pm), pm.p);
         __syncthreads();
         cur[t] = r;
     }
@@ -257,34 +247,20 @@ __global__ __launch_bounds__(1 << kEvalFuse) void k_inv_bottom(
         u32 *g0 = nxt + c * (2 * h);
         if (v < h) {
             const u32 *ia = cur + (2 * c) * h, *ib = ia + h;
-            u64 acc = 0;
-            for (int s2 = 0; s2 <= v; ++s2) {
-                acc += (u64)ia[s2] * ib[v - s2];
-                if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
-            }
-            g0[v] = plain_reduce(acc, pm);
+            g0[v] = plain_dot<1, -1>(ia, ib + v, v + 1, pm);
         }
         __syncthreads();
         if (v >= h) {
             const u32 *M = Ml + c * (2 * h + 1);
-            u64 acc = 0;
-            for (int s2 = v - h + 1; s2 <= v; ++s2) {
-                acc += (u64)M[2 * h - s2] * g0[v - s2];
-                if (((s2 - v + h - 1) & 3) == 3) acc = plain_fold(acc, pm.c32);
-            }
-            const u32 e = plain_reduce(acc, pm);
+            // s = v - h + 1 + k: M_(2h - s) g0_(v - s) = M_(3h-1-v-k) g0_(h-1-k)
+            const u32 e = plain_dot<-1, -1>(M + (3 * h - 1 - v), g0 + (h - 1), h, pm);
             tmp[c * h + (v - h)] = e ? p - e : 0u;
         }
         __syncthreads();
         if (v >= h) {
             const int w = v - h;
             const u32 *me = tmp + c * h;
-            u64 acc = 0;
-            for (int s2 = 0; s2 <= w; ++s2) {
-                acc += (u64)g0[s2] * me[w - s2];
-                if ((s2 & 3) == 3) acc = plain_fold(acc, pm.c32);
-            }
-            g0[v] = plain_reduce(acc, pm);
+            g0[v] = plain_dot<1, -1>(g0, me + w, w + 1, pm);
         }
         __syncthreads();
         cur[t] = nxt[t];

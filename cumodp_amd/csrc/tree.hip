@@ -87,12 +87,7 @@ __global__ __launch_bounds__(1 << kTreeFuse) void k_tree_bottom(const u32 *__res
             const int c = x / len, r = x - c * len;
             const u32 *a = prev + (2 * c) * (h + 1), *b = a + (h + 1);
             const int lo = r > h ? r - h : 0, hi = r < h ? r : h;
-            u64 acc = 0;
-            for (int u = lo; u <= hi; ++u) {
-                acc += (u64)a[u] * b[r - u];
-                if (((u - lo) & 3) == 3) acc = plain_fold(acc, pm.c32);
-            }
-            const u32 v = plain_reduce(acc, pm);
+            const u32 v = plain_dot<1, -1>(a + lo, b + (r - lo), hi - lo + 1, pm);
             cur[x] = v;
             out[x] = v;
         }
