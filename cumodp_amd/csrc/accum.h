@@ -52,6 +52,15 @@ __device__ __forceinline__ u32 plain_dot(const u32 *x, const u32 *y, int cnt,
     return plain_reduce(acc, m);
 }
 
+// Part `part` of PARTS of the same sum: indices [cnt part / PARTS,
+// cnt (part + 1) / PARTS).  The parts' residues add up (mod p) to the sum.
+template <int DX, int DY, int PARTS>
+__device__ __forceinline__ u32 plain_dot_part(const u32 *x, const u32 *y, int cnt, int part,
+                                              const PlainMod &m) {
+    const int k0 = cnt * part / PARTS, k1 = cnt * (part + 1) / PARTS;
+    return plain_dot<DX, DY>(x + k0 * DX, y + k0 * DY, k1 - k0, m);
+}
+
 // host: the constants for modulus p (odd, 3 <= p < 2^31)
 static inline PlainMod plain_mod(u32 p) {
     PlainMod m;

@@ -150,8 +150,8 @@ __global__ void k_horner(const u32 *__restrict__ R, int h, const u32 *__restrict
 
 // The lowest remainder levels and the leaves in one launch.  Workgroup J
 // takes node J of level `pin` (its remainder at A + J 2^pin, blockDim.x =
-// 2^pin threads, pin <= kEvalFuse) and descends its subtree to level L0 in
-// LDS: at level i (h = 2^i) thread t = c h + u computes coefficient u of
+// kEvalParts 2^pin threads, pin <= kEvalFuse) and descends its subtree to
+// level L0 in LDS: at level i (h = 2^i) slot t = c h + u computes coefficient u of
 // node c, first qrev_u = sum_(s<=u) A'_(2h-1-s) I_(u-s) (A' the parent's
 // remainder, I = I_i(node)), then q_s = qrev_(h-1-s) and
 // R_u = A'_u - sum_(s<=u) q_s M_(u-s) -- the same division as the level-wide
@@ -160,35 +160,52 @@ __global__ void k_horner(const u32 *__restrict__ R, int h, const u32 *__restrict
 // Horner's rule, as k_horner does.
 constexpr int kEvalFuse = 8;
 
-__global__ __launch_bounds__(1 << kEvalFuse) void k_rem_descend(
+// Each sum is split over kEvalParts threads (thread = part 2^pin + t) whose
+// residues thread t adds up: 4 2^pin threads per workgroup.
+constexpr int kEvalParts = 4;
+static_assert(kEvalParts == 4, "the partial sums are added as four terms");
+
+__global__ __launch_bounds__(kEvalParts << kEvalFuse) void k_rem_descend(
     const u32 *__restrict__ A, int pin, int L0, const u32 *__restrict__ tree, LevelOffsets lo,
     const u32 *__restrict__ inv, int64_t n, const u32 *__restrict__ pts, PlainMod pm,
     u32 *__restrict__ val) {
     constexpr int BMAX = 1 << kEvalFuse;
-    __shared__ u32 cur[BMAX], qb[BMAX], Il[BMAX], Ml[BMAX + (BMAX >> 4)];
+    __shared__ u32 cur[BMAX], qb[BMAX], Il[BMAX], Ml[BMAX + (BMAX >> 4)], pp[kEvalParts * BMAX];
     const int B = 1 << pin;
     const int64_t J = blockIdx.x;
-    const int t = threadIdx.x;
-    cur[t] = A[J * B + t];
+    const int tid = threadIdx.x, t = tid & (B - 1), part = tid >> pin;
+    if (part == 0) cur[t] = A[J * B + t];
     for (int i = pin - 1; i >= L0; --i) {
         const int h = 1 << i, nodes = B >> i;
         const int c = t >> i, u = t & (h - 1);
         // this level's inverses (B words) and node polynomials (nodes (h+1)
         // words) of the subtree, both contiguous in global memory
-        Il[t] = inv[(int64_t)(i - L0) * n + J * B + t];
+        if (part == 0) Il[t] = inv[(int64_t)(i - L0) * n + J * B + t];
         const u32 *Mg = tree + lo.off[i] + (J * nodes) * (h + 1);
-        for (int x = t; x < nodes * (h + 1); x += B) Ml[x] = Mg[x];
+        for (int x = tid; x < nodes * (h + 1); x += kEvalParts * B) Ml[x] = Mg[x];
         __syncthreads();
         const u32 *par = cur + (c >> 1) * (2 * h);
         const u32 *I = Il + c * h;
-        qb[t] = plain_dot<-1, -1>(par + 2 * h - 1, I + u, u + 1, pm);
+        pp[tid] = plain_dot_part<-1, -1, kEvalParts>(par + 2 * h - 1, I + u, u + 1, part, pm);
+        __syncthreads();
+        if (part == 0)
+            qb[t] = cm_add(cm_add(pp[t], pp[B + t], pm.p), cm_add(pp[2 * B + t], pp[3 * B + t], pm.p),
+                           pm.p);
         __syncthreads();
         const u32 *q = qb + c * h, *M = Ml + c * (h + 1);
-        const u32 r = cm_sub(par[u], plain_dot<-1, -1>(q + h - 1, M + u, u + 1, pm), pm.p);
+        pp[tid] = plain_dot_part<-1, -1, kEvalParts>(q + h - 1, M + u, u + 1, part, pm);
         __syncthreads();
-        cur[t] = r;
+        u32 r = 0;
+        if (part == 0) {
+            const u32 qm = cm_add(cm_add(pp[t], pp[B + t], pm.p),
+                                  cm_add(pp[2 * B + t], pp[3 * B + t], pm.p), pm.p);
+            r = cm_sub(par[u], qm, pm.p);
+        }
+        __syncthreads();                      // every read of cur (par) is done
+        if (part == 0) cur[t] = r;
+        __syncthreads();
     }
-    __syncthreads();
+    if (part != 0) return;
     const int64_t x = J * B + t;
     const u32 *r = cur + ((t >> L0) << L0);
     const u32 zr = cm_mul_tw(pts[x] % pm.p, pm.r2, pm.p, pm.np);   // z 2^32 mod p
@@ -199,25 +216,29 @@ __global__ __launch_bounds__(1 << kEvalFuse) void k_rem_descend(
 }
 
 // The lowest subinverse levels in one launch: workgroup J (blockDim.x =
-// 2^pin threads, kEvalBase < pin <= kEvalFuse) takes the nodes of level
+// kEvalParts 2^pin threads, kEvalBase < pin <= kEvalFuse) takes the nodes of level
 // kEvalBase .. pin under node J of level pin.  The base level comes from the
 // power-series recurrence of k_inv_base (one thread per node); each level
 // above from the Newton step of the level-wide launches, with the three
-// products as schoolbook sums in LDS: thread t = c 2h + v (node c of level
+// products as schoolbook sums in LDS: slot t = c 2h + v (node c of level
 // i+1, h = 2^i) forms g0_v (v < h), then -E_(v-h) = -sum f_s g0_(v-s)
 // (v >= h, f_s = M_(2h-s)), then the high half sum g0_s (-E)_(v-h-s).
 // Every level is written to the subinverse tree.
-__global__ __launch_bounds__(1 << kEvalFuse) void k_inv_bottom(
+__global__ __launch_bounds__(kEvalParts << kEvalFuse) void k_inv_bottom(
     const u32 *__restrict__ tree, LevelOffsets lo, int pin, PlainMod pm, u32 *__restrict__ inv,
     int64_t n) {
     constexpr int BMAX = 1 << kEvalFuse, H0 = 1 << kEvalBase;
-    __shared__ u32 cur[BMAX], nxt[BMAX], tmp[BMAX / 2], Ml[BMAX + BMAX / (2 * H0)];
+    __shared__ u32 cur[BMAX], nxt[BMAX], tmp[BMAX / 2], Ml[BMAX + BMAX / (2 * H0)],
+        pp[kEvalParts * BMAX];
     const int B = 1 << pin;
     const int64_t J = blockIdx.x;
-    const int t = threadIdx.x;
+    const int tid = threadIdx.x, t = tid & (B - 1), part = tid >> pin;
     const u32 p = pm.p;
+    auto sum4 = [&](int t2) {
+        return cm_add(cm_add(pp[t2], pp[B + t2], p), cm_add(pp[2 * B + t2], pp[3 * B + t2], p), p);
+    };
     const int nb = B >> kEvalBase;
-    if (t < nb) {
+    if (part == 0 && t < nb) {
         const u32 *m = tree + lo.off[kEvalBase] + (J * nb + t) * (H0 + 1);
         u32 f[H0], g[H0];
 #pragma unroll
@@ -238,33 +259,43 @@ __global__ __launch_bounds__(1 << kEvalFuse) void k_inv_bottom(
         for (int w = 0; w < H0; ++w) cur[t * H0 + w] = g[w];
     }
     __syncthreads();
-    inv[J * B + t] = cur[t];
+    if (part == 0) inv[J * B + t] = cur[t];
     for (int i = kEvalBase; i < pin; ++i) {
         const int h = 1 << i, nodes = B >> (i + 1);
         const int c = t >> (i + 1), v = t & (2 * h - 1);
         const u32 *Mg = tree + lo.off[i + 1] + (J * nodes) * (2 * h + 1);
-        for (int x = t; x < nodes * (2 * h + 1); x += B) Ml[x] = Mg[x];
+        for (int x = tid; x < nodes * (2 * h + 1); x += kEvalParts * B) Ml[x] = Mg[x];
         u32 *g0 = nxt + c * (2 * h);
         if (v < h) {
             const u32 *ia = cur + (2 * c) * h, *ib = ia + h;
-            g0[v] = plain_dot<1, -1>(ia, ib + v, v + 1, pm);
+            pp[tid] = plain_dot_part<1, -1, kEvalParts>(ia, ib + v, v + 1, part, pm);
         }
+        __syncthreads();
+        if (part == 0 && v < h) g0[v] = sum4(t);
         __syncthreads();
         if (v >= h) {
             const u32 *M = Ml + c * (2 * h + 1);
             // s = v - h + 1 + k: M_(2h - s) g0_(v - s) = M_(3h-1-v-k) g0_(h-1-k)
-            const u32 e = plain_dot<-1, -1>(M + (3 * h - 1 - v), g0 + (h - 1), h, pm);
+            pp[tid] = plain_dot_part<-1, -1, kEvalParts>(M + (3 * h - 1 - v), g0 + (h - 1), h,
+                                                         part, pm);
+        }
+        __syncthreads();
+        if (part == 0 && v >= h) {
+            const u32 e = sum4(t);
             tmp[c * h + (v - h)] = e ? p - e : 0u;
         }
         __syncthreads();
         if (v >= h) {
             const int w = v - h;
-            const u32 *me = tmp + c * h;
-            g0[v] = plain_dot<1, -1>(g0, me + w, w + 1, pm);
+            pp[tid] = plain_dot_part<1, -1, kEvalParts>(g0, tmp + c * h + w, w + 1, part, pm);
         }
         __syncthreads();
-        cur[t] = nxt[t];
-        inv[(int64_t)(i + 1 - kEvalBase) * n + J * B + t] = nxt[t];
+        if (part == 0 && v >= h) g0[v] = sum4(t);   // high half; the sums read the low half
+        __syncthreads();
+        if (part == 0) {
+            cur[t] = nxt[t];
+            inv[(int64_t)(i + 1 - kEvalBase) * n + J * B + t] = nxt[t];
+        }
         __syncthreads();
     }
 }
@@ -298,7 +329,7 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     if (pinv > L0) {
         LevelOffsets lo{};
         for (int l = 0; l <= k && l < 32; ++l) lo.off[l] = tree_level_offset(k, l);
-        hipLaunchKernelGGL(k_inv_bottom, dim3((unsigned)(n >> pinv)), dim3(1 << pinv), 0, s,
+        hipLaunchKernelGGL(k_inv_bottom, dim3((unsigned)(n >> pinv)), dim3(kEvalParts << pinv), 0, s,
                            tree, lo, pinv, plain_mod(p), inv, n);
         e = hipGetLastError();
         istart = pinv;
@@ -393,7 +424,7 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     if (fuse) {
         LevelOffsets lo{};
         for (int l = 0; l <= k && l < 32; ++l) lo.off[l] = tree_level_offset(k, l);
-        hipLaunchKernelGGL(k_rem_descend, dim3((unsigned)(n >> pin)), dim3(1 << pin), 0, s, A,
+        hipLaunchKernelGGL(k_rem_descend, dim3((unsigned)(n >> pin)), dim3(kEvalParts << pin), 0, s, A,
                            pin, L0, tree, lo, inv, n, pts, plain_mod(p), values);
         return hipGetLastError();
     }
