@@ -22,6 +22,24 @@ def test_fast_evaluation_exact(cm, orc, k):
     assert (got == orc.poly_eval(F, pts, p)).all()
 
 
+@pytest.mark.parametrize("k", [6, 11, 13])
+def test_fast_evaluation_exact_31bit(cm, orc, k):
+    """The same at p = 2013265921 (above 2^30: the canonical butterflies;
+    the fused tree, subinverse and remainder kernels at full 31-bit width),
+    with coefficients and points drawn near p."""
+    p = 2013265921
+    rng = np.random.default_rng(900 + k)
+    n = 1 << k
+    pts = (p - 1 - rng.integers(0, 1 << 20, n)).astype(np.int64).astype(np.int32)
+    pts = np.unique(pts)
+    while len(pts) < n:                                   # distinct points, topped up
+        pts = np.unique(np.concatenate([pts, rng.integers(0, p, n).astype(np.int32)]))
+    pts = pts[:n]
+    F = (p - 1 - rng.integers(0, 3, n)).astype(np.int64).astype(np.int32)
+    got = cm.fast_evaluation(F, pts, p)
+    assert (got == orc.poly_eval(F, pts, p)).all()
+
+
 def test_fast_evaluation_2e16_sampled(cm, orc):
     w = cm.workload4_tree()
     p, pts = w["p"], w["points"]
