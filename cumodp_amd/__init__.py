@@ -60,6 +60,8 @@ CXX_SYMBOLS = {
     "bivariate_stockham_host": "_Z23bivariate_stockham_hostPiiiiii",
     "inverse_bivariate_stockham_host": "_Z31inverse_bivariate_stockham_hostPiiiiii",
     "bi_stockham_poly_mul_dev": "_Z24bi_stockham_poly_mul_deviiiiPiS_i",
+    "list_bivariate_stockham_dev_W": "_Z27list_bivariate_stockham_devPiiiPKiiS1_i",
+    "list_bivariate_stockham_dev_w": "_Z27list_bivariate_stockham_devPiiiiiii",
     "mulCUDA": "_Z7mulCUDAPiS_S_iiii",
 }
 

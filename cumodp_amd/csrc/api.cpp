@@ -215,18 +215,27 @@ cumodp_err ntt_host_impl(sfixn *X, int64_t m, int k, u32 w, u32 p, bool inverse)
 // DFT_{m,n} on a row-major 2^em x 2^en device array: rows with root wn,
 // then columns with root wm (src/list_stockham.cu:1161-1204); the inverse
 // uses the inverse roots and scales by (mn)^-1.
+// q such arrays, one after the other (I_q (x) DFT_{m,n},
+// src/list_stockham.cu:1803-1846): the row transforms of all q m rows in one
+// call, the column transforms of all q arrays in one launch per 65535 arrays.
 cumodp_err biv_dev_impl(u32 *X, int em, u32 wm, int en, u32 wn, u32 p, bool inverse,
-                        hipStream_t s) {
+                        hipStream_t s, int64_t q = 1) {
     if (em < 1 || em > kMaxLdsLog) return fail(CUMODP_FFT_SIZE_TOO_LARGE, "2^em must lie in [2, 2^13]");
+    if (q < 1) return fail(CUMODP_ASSUMPTION_ERROR, "need at least one array");
     cumodp_err ce = check_prime_size(p, em);
     if (ce != CUMODP_SUCCESS) return ce;
     if (inverse) { wm = cm_host_inv(wm % p, p); wn = cm_host_inv(wn % p, p); }
-    ce = ntt_dev_impl(X, (int64_t)1 << em, en, wn, p, false, s);
+    ce = ntt_dev_impl(X, q << em, en, wn, p, false, s);
     if (ce != CUMODP_SUCCESS) return ce;
     const Family *fm = family_for_root(p, wm % p, em);
     if (!fm) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
     const u32 sc = inverse ? cm_host_inv(cm_host_pow(2, em + en, p), p) : 1u;
-    hipError_t he = launch_col_ntt(em, X, (int64_t)1 << en, sc, fm, s);
+    const int64_t words = (int64_t)1 << (em + en);
+    hipError_t he = hipSuccess;
+    for (int64_t b0 = 0; b0 < q && he == hipSuccess; b0 += 65535) {
+        const int nb = (int)std::min<int64_t>(65535, q - b0);
+        he = launch_col_ntt(em, X + b0 * words, (int64_t)1 << en, sc, fm, s, nb, words);
+    }
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "bivariate transform", he);
     return CUMODP_SUCCESS;
@@ -259,6 +268,10 @@ u32 read_root(const sfixn *W) {
     (void)hipMemcpy(&w, W + 1, sizeof(w), hipMemcpyDeviceToHost);
     return (u32)w;
 }
+
+// the root of order 2^k of a power table [1, w, ..., w^(2^(k-1)-1)]: W[1],
+// except for k = 1, whose one-word table [1] leaves w = -1 implied
+u32 read_root_k(const sfixn *W, int k, u32 p) { return k == 1 ? p - 1 : read_root(W); }
 
 void report(const char *fn, cumodp_err e) {
     if (e != CUMODP_SUCCESS) fprintf(stderr, "%s: %s\n", fn, g_last_error.c_str());
@@ -367,6 +380,14 @@ cumodp_err build_cuda_scube(void *, const sfixn *, const sfixn *, const sfixn *,
 }
 const sfixn *interp_subres_coeff2(sfixn *, void *, sfixn, sfixn) { return nullptr; }
 const sfixn *interp_subres_coeff3(sfixn *, sfixn *, void *, sfixn, sfixn) { return nullptr; }
+float gcdPrimeField(sfixn *, sfixn *, sfixn *, sfixn *, sfixn, sfixn, sfixn, sfixn) {
+    fprintf(stderr, "gcdPrimeField: not provided by this library (outputs untouched)\n");
+    return -1.0f;
+}
+float divPrimeField(sfixn *, sfixn *, sfixn *, sfixn *, sfixn, sfixn, sfixn) {
+    fprintf(stderr, "divPrimeField: not provided by this library (outputs untouched)\n");
+    return -1.0f;
+}
 
 // ===========================================================================
 // C linkage: include/cumodp_amd.h
@@ -694,6 +715,7 @@ struct CapturedGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<void *> host_bufs;   // pinned descriptor copies the replays read
+    std::vector<void *> scratch;     // device scratch the replays use (held, see plan.cpp)
 };
 }  // namespace
 
@@ -724,6 +746,7 @@ void *cumodp_graph_end(void *stream) {
     cg->graph = g;
     cg->exec = x;
     cg->host_bufs.swap(bufs);
+    cg->scratch = hold_workspace_for_graph(s);
     return cg;
 }
 
@@ -741,6 +764,7 @@ cumodp_err cumodp_graph_destroy(void *graph) {
     if (cg->exec) (void)hipGraphExecDestroy(cg->exec);
     if (cg->graph) (void)hipGraphDestroy(cg->graph);
     for (void *h : cg->host_bufs) (void)hipHostFree(h);
+    release_graph_workspace(cg->scratch);
     delete cg;
     return CUMODP_SUCCESS;
 }
@@ -990,7 +1014,8 @@ void reset_vector_dev(sfixn n, sfixn *X) {
 void bivariate_stockham_dev(sfixn *X, sfixn em, const sfixn *Wm, sfixn en, const sfixn *Wn,
                             sfixn p) {
     report("bivariate_stockham_dev",
-           biv_dev_impl((u32 *)X, em, read_root(Wm), en, read_root(Wn), (u32)p, false, 0));
+           biv_dev_impl((u32 *)X, em, read_root_k(Wm, em, (u32)p), en,
+                        read_root_k(Wn, en, (u32)p), (u32)p, false, 0));
 }
 
 void inverse_bivariate_stockham_dev(sfixn *X, sfixn em, const sfixn *Wm, sfixn en,
@@ -998,8 +1023,8 @@ void inverse_bivariate_stockham_dev(sfixn *X, sfixn em, const sfixn *Wm, sfixn e
     // src/list_stockham.cu:1251-1263: the tables already hold inverse roots
     const u32 P = (u32)p;
     report("inverse_bivariate_stockham_dev",
-           biv_dev_impl((u32 *)X, em, cm_host_inv(read_root(Wm) % P, P), en,
-                        cm_host_inv(read_root(Wn) % P, P), P, true, 0));
+           biv_dev_impl((u32 *)X, em, cm_host_inv(read_root_k(Wm, em, P) % P, P), en,
+                        cm_host_inv(read_root_k(Wn, en, P) % P, P), P, true, 0));
 }
 
 void bivariate_stockham_host(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, sfixn p) {
@@ -1009,6 +1034,20 @@ void bivariate_stockham_host(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, s
 void inverse_bivariate_stockham_host(sfixn *X, sfixn em, sfixn wm, sfixn en, sfixn wn, sfixn p) {
     report("inverse_bivariate_stockham_host",
            biv_host_impl(X, em, (u32)wm, en, (u32)wn, (u32)p, true));
+}
+
+void list_bivariate_stockham_dev(sfixn *X, sfixn q, sfixn em, sfixn wm, sfixn en, sfixn wn,
+                                 sfixn p) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    report("list_bivariate_stockham_dev",
+           biv_dev_impl((u32 *)X, em, (u32)wm, en, (u32)wn, (u32)p, false, 0, q));
+}
+
+void list_bivariate_stockham_dev(sfixn *X, sfixn q, sfixn em, const sfixn *Wm, sfixn en,
+                                 const sfixn *Wn, sfixn p) {
+    // the tables are [1, w, ..., w^(m/2-1)] (src/list_stockham.cu:1849-1862)
+    list_bivariate_stockham_dev(X, q, em, (sfixn)read_root_k(Wm, em, (u32)p), en,
+                                (sfixn)read_root_k(Wn, en, (u32)p), p);
 }
 
 void bi_stockham_poly_mul_dev(sfixn m, sfixn em, sfixn n, sfixn en, sfixn *F, sfixn *G,

@@ -524,12 +524,14 @@ __global__ __launch_bounds__((Tile<LOGL2, CR, false>::T)) void k_row_out(
 
 // Natural-order column transforms DFT_m (x) I_ncols of a row-major
 // 2^LOGM x ncols array, in place (the column half of the bivariate DFT,
-// src/list_stockham.cu:1193-1201); output scaled by sc when sc.x != 1.
+// src/list_stockham.cu:1193-1201); output scaled by sc when sc != 0.
 template <int LOGM, int C>
 __global__ __launch_bounds__((Tile<LOGM, C, true>::T)) void k_col_ntt(
-    u32 *__restrict__ X, int64_t ncols, const u32 *__restrict__ lv, Mod md, u32 sc) {
+    u32 *__restrict__ X0, int64_t ncols, int64_t bstride, const u32 *__restrict__ lv, Mod md,
+    u32 sc) {
     using TL = Tile<LOGM, C, true>;
     __shared__ u32 lds[TL::LDS_WORDS];
+    u32 *X = X0 + (int64_t)blockIdx.y * bstride;   // grid.y: arrays bstride words apart
     const int64_t c0 = (int64_t)blockIdx.x * C;
     constexpr int PER = TL::WORDS / TL::T;
     u32 xa[PER];
@@ -906,6 +908,14 @@ hipError_t launch_mul_four_step(int k, const FourStepIO &io, const Family *fam, 
     return mul_four_step_t(k1, k2, io, o, fam, mod_of(fam), s);
 }
 
+hipError_t launch_mul_four_step_with(const FourStepIO &io, const Family *fam, const Outer *o,
+                                     hipStream_t s) {
+    if (!o || o->k > fam->K || o->k <= kMaxLdsLog) return hipErrorInvalidValue;
+    if (g_col_direct && fam->p < (1u << 30))
+        return mul_four_step_t(o->k1, o->k2, io, o, fam, ModLazy{fam->p, fam->np}, s);
+    return mul_four_step_t(o->k1, o->k2, io, o, fam, mod_of(fam), s);
+}
+
 template <int LOGL>
 static hipError_t rows_mul_t(u32 *F, u32 *G, int rows, const Family *fam, hipStream_t s) {
     const int L = 1 << LOGL;
@@ -991,21 +1001,23 @@ hipError_t launch_ntt_four_step(int k, u32 *X, u32 *Y, int cnt, bool inverse, co
 }
 
 template <int LOGM>
-static hipError_t col_ntt_t(u32 *X, int64_t ncols, u32 sc, const Family *fam, hipStream_t s) {
+static hipError_t col_ntt_t(u32 *X, int64_t ncols, int nblk, int64_t bstride, u32 sc,
+                            const Family *fam, hipStream_t s) {
     constexpr int C = col_tile<LOGM>();
     using TL = Tile<LOGM, C, true>;
     const int64_t nb = (ncols + C - 1) / C;
     ProfScope ps(kProfColNtt, s);
-    hipLaunchKernelGGL((k_col_ntt<LOGM, C>), dim3((unsigned)nb), dim3(TL::T), 0, s, X, ncols,
-                       fam->lv, mod_of(fam), sc);
+    hipLaunchKernelGGL((k_col_ntt<LOGM, C>), dim3((unsigned)nb, (unsigned)nblk), dim3(TL::T), 0,
+                       s, X, ncols, bstride, fam->lv, mod_of(fam), sc);
     return hipGetLastError();
 }
 
 hipError_t launch_col_ntt(int em, u32 *X, int64_t ncols, u32 scale, const Family *fam,
-                          hipStream_t s) {
+                          hipStream_t s, int nblk, int64_t bstride) {
+    if (nblk <= 0 || nblk > 65535) return hipErrorInvalidValue;
     const int k1 = em;
     const u32 sc = scale_mont(scale, fam->p);
-    CM_K1_CASES(col_ntt_t, X, ncols, sc, fam, s)
+    CM_K1_CASES(col_ntt_t, X, ncols, nblk, bstride, sc, fam, s)
 }
 
 hipError_t launch_pointwise(u32 *Y, const u32 *X, int64_t n, u32 p, hipStream_t s) {

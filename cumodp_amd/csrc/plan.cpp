@@ -107,6 +107,12 @@ const Family *family_default(u32 p) {
     return f;
 }
 
+const Family *family_default_cached(u32 p) {
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto it = g_default.find(p);
+    return it != g_default.end() ? it->second : nullptr;
+}
+
 void four_step_split(int k, int &k1, int &k2) {
     int c = (k >= 0 && k <= kMaxLog) ? g_split[k] : 0;
     if (!c) {
@@ -146,6 +152,8 @@ const Outer *outer_for(const Family *f, int k) {
     size_t nlo = (size_t)1 << kOuterLoBits, nhi = (size_t)1 << (k - kOuterLoBits);
     Outer *o = new Outer();
     o->k = k;
+    o->k1 = k1;
+    o->k2 = k2;
     o->lo_f = upload(power_table(wn, nlo, p));
     o->hi_f = upload(power_table(cm_host_pow(wn, nlo, p), nhi, p));
     o->lo_i = upload(power_table(wni, nlo, p));
@@ -184,9 +192,36 @@ const Outer *outer_for(const Family *f, int k) {
     return o;
 }
 
+const Outer *outer_cached(const Family *f, int k) {
+    int k1, k2;
+    four_step_split(k, k1, k2);
+    std::lock_guard<std::mutex> lk(g_plan_mutex);
+    auto it = g_outer.find(std::make_tuple(f, k, k2));
+    return it != g_outer.end() ? it->second : nullptr;
+}
+
 namespace {
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, std::pair<void *, size_t>> g_ws;
+// Scratch buffers that captured graphs refer to: buffer -> number of live
+// graphs.  Such a buffer is never freed while a graph holds it; when its
+// stream needs a larger one (or is destroyed) it is only detached
+// (g_ws_detached) and freed when the last graph holding it is destroyed.
+std::map<void *, int> g_ws_graph_refs;
+std::map<void *, bool> g_ws_detached;
+
+// drops buffer b from its (slot, stream) entry: freed at once unless a live
+// graph refers to it (caller holds g_ws_mutex; work queued on the stream
+// that may still use b has been synchronised)
+void retire_buffer_locked(void *b) {
+    if (!b) return;
+    auto r = g_ws_graph_refs.find(b);
+    if (r != g_ws_graph_refs.end() && r->second > 0) {
+        g_ws_detached[b] = true;
+        return;
+    }
+    (void)hipFree(b);
+}
 }  // namespace
 
 bool stream_capturing(hipStream_t s) {
@@ -206,7 +241,7 @@ void *workspace(int slot, size_t bytes, hipStream_t s) {
     if (b.first) {
         // work already queued on s may still use the old buffer
         (void)hipStreamSynchronize(s);
-        (void)hipFree(b.first);
+        retire_buffer_locked(b.first);
         b.first = nullptr;
         b.second = 0;
     }
@@ -320,6 +355,41 @@ std::vector<void *> take_captured_buffers(hipStream_t s) {
     return v;
 }
 
+std::vector<void *> hold_workspace_for_graph(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    std::vector<hipStream_t> streams{s};
+    auto a = g_aux.find(s);
+    if (a != g_aux.end())
+        for (int i = 0; i < kAuxStreams; ++i) streams.push_back(a->second->s[i]);
+    std::vector<void *> held;
+    for (auto &kv : g_ws) {
+        if (!kv.second.first) continue;
+        for (hipStream_t t : streams) {
+            if (kv.first.second == t) {
+                held.push_back(kv.second.first);
+                ++g_ws_graph_refs[kv.second.first];
+                break;
+            }
+        }
+    }
+    return held;
+}
+
+void release_graph_workspace(const std::vector<void *> &held) {
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    for (void *b : held) {
+        auto r = g_ws_graph_refs.find(b);
+        if (r == g_ws_graph_refs.end()) continue;
+        if (--r->second > 0) continue;
+        g_ws_graph_refs.erase(r);
+        auto d = g_ws_detached.find(b);
+        if (d != g_ws_detached.end()) {
+            g_ws_detached.erase(d);
+            (void)hipFree(b);
+        }
+    }
+}
+
 void release_workspace(hipStream_t s) {
     AuxStreams *a = nullptr;
     {
@@ -343,7 +413,7 @@ void release_workspace(hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_ws_mutex);
     for (auto it = g_ws.begin(); it != g_ws.end();) {
         if (it->first.second == s) {
-            if (it->second.first) (void)hipFree(it->second.first);
+            retire_buffer_locked(it->second.first);
             it = g_ws.erase(it);
         } else {
             ++it;

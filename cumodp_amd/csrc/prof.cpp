@@ -72,12 +72,17 @@ void drain_locked() {
 
 }  // namespace
 
+thread_local bool t_suspended = false;
+
+void prof_suspend(bool on) { t_suspended = on; }
+
 bool prof_on() {
     std::lock_guard<std::mutex> lk(g_m);
     return enabled_locked();
 }
 
 ProfScope::ProfScope(int id, hipStream_t s) : id_(-1), s_(s) {
+    if (t_suspended) return;
     std::lock_guard<std::mutex> lk(g_m);
     if (!enabled_locked() || id < 0 || id >= kProfCount) return;
     if (stream_capturing(s)) return;   // graphs are timed as a whole by the caller

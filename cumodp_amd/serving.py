@@ -17,6 +17,16 @@ Design:
     one host-to-device copy in and one device-to-host copy out per batch.
   * Admission by HBM footprint. A batch takes requests while its input,
     output and estimated transform scratch bytes stay under the slot's cap.
+  * Admission by service time. A batch also stops growing once its
+    estimated GPU time reaches `max_batch_s`. The estimate is a work count
+    per request (transform scratch words times log2 of the y-transform
+    length) times seconds per unit, learnt from the batches already served:
+    each retired batch's service time (from the later of its launch and the
+    previous batch's completion to its own completion) over its work count,
+    as an exponentially weighted average. Without this cap a burst makes
+    one batch of hundreds of requests, every request in it waits for all
+    of them, and the arrivals during that long batch make the next one as
+    long.
   * Twiddle tables stay resident in the library across batches.
 
 The reference has no serving layer. This drives the integer multiply that
@@ -130,6 +140,8 @@ class _Slot:
         self.din = self.dout = None
         self.cap_in = self.cap_out = 0
         self.range = None            # (i0, i1) in flight
+        self.units = 0.0             # its work count
+        self.t_launch = 0.0          # when it was launched (seconds into the run)
 
     def ensure(self, nin, nout):
         L = lib()
@@ -159,11 +171,14 @@ class _Slot:
 
 class ZmulServer:
     """Continuous batching over `slots` streams; a batch holds at most
-    `slot_bytes` of inputs, outputs and estimated transform scratch."""
+    `slot_bytes` of inputs, outputs and estimated transform scratch, and at
+    most `max_batch_s` seconds of estimated GPU work (None: no time cap)."""
 
-    def __init__(self, slots=2, slot_bytes=1 << 30):
+    def __init__(self, slots=2, slot_bytes=1 << 30, max_batch_s=None):
         self.slots = [_Slot() for _ in range(slots)]
         self.slot_bytes = int(slot_bytes)
+        self.max_batch_s = max_batch_s
+        self.sec_per_unit = None     # learnt from served batches
         self._plans = {}
 
     def close(self):
@@ -186,18 +201,39 @@ class ZmulServer:
             self._plans[key] = v
         return v
 
+    def _units(self, bits, ny):
+        """Work count of one request: its transform scratch words times log2
+        of its y-transform length (proportional to its butterflies)."""
+        ey = max(4, (int(ny) - 1).bit_length())
+        return self._scratch_bytes(bits, ny) * ey / 4.0
+
     def _fit(self, Q, i0, i1):
-        """Largest j in (i0, i1] such that requests [i0, j) fit one slot."""
+        """Largest j in (i0, i1] such that requests [i0, j) fit one slot: the
+        footprint cap and, once the rate is known, the service-time cap."""
         total = 0
+        work = 0.0
+        cap_units = None
+        if self.max_batch_s is not None and self.sec_per_unit:
+            cap_units = self.max_batch_s / self.sec_per_unit
         j = i0
         while j < i1:
             b = 8 * (int(Q.na[j] + Q.nb[j]) + int(Q.nc[j] * Q.wc[j])) + \
                 self._scratch_bytes(Q.bits[j], Q.nc[j])
-            if j > i0 and total + b > self.slot_bytes:
+            u = self._units(Q.bits[j], Q.nc[j])
+            if j > i0 and (total + b > self.slot_bytes or
+                           (cap_units is not None and work + u > cap_units)):
                 break
             total += b
+            work += u
             j += 1
         return j
+
+    def _learn(self, units, seconds):
+        """Fold one retired batch's measured service time into the rate."""
+        if units <= 0 or seconds <= 0:
+            return
+        r = seconds / units
+        self.sec_per_unit = r if self.sec_per_unit is None else 0.8 * self.sec_per_unit + 0.2 * r
 
     def _launch(self, slot, Q, i0, i1):
         L = lib()
@@ -218,6 +254,7 @@ class ZmulServer:
         _check("cumodp_memcpy_d2h_async", L.cumodp_memcpy_d2h_async(
             C.c_void_p(Q.OUT.ptr + 8 * out0), C.c_void_p(slot.dout), 8 * (out1 - out0), st))
         slot.range = (i0, i1)
+        slot.units = sum(self._units(Q.bits[r], Q.nc[r]) for r in range(i0, i1))
 
     def run(self, Q, arrivals):
         """Serve requests 0..Q.n-1 arriving at `arrivals` (seconds from the
@@ -233,6 +270,7 @@ class ZmulServer:
         batches = []
         t0 = time.perf_counter()
         retired = 0
+        last_done = 0.0
         while retired < N:
             now = time.perf_counter() - t0
             # retire finished batches in launch order
@@ -247,12 +285,17 @@ class ZmulServer:
                 i0, i1 = s.range
                 done[i0:i1] = t
                 retired += i1 - i0
+                # its GPU time began when it was launched or when the batch
+                # ahead of it finished, whichever was later
+                self._learn(s.units, t - max(s.t_launch, last_done))
+                last_done = t
                 s.range = None
                 free.append(s)
             arrived = int(np.searchsorted(arrivals, now, side="right"))
             if head < arrived and free:
                 j = self._fit(Q, head, arrived)
                 s = free.pop(0)
+                s.t_launch = time.perf_counter() - t0
                 self._launch(s, Q, head, j)
                 batches.append(j - head)
                 inflight.append(s)

@@ -73,6 +73,15 @@ CUMODP_API cumodp_err build_cuda_scube(void *scb, const sfixn *sz_p, const sfixn
 CUMODP_API const sfixn *interp_subres_coeff2(sfixn *nx, void *scb, sfixn i, sfixn j);
 CUMODP_API const sfixn *interp_subres_coeff3(sfixn *nx, sfixn *ny, void *scb, sfixn i, sfixn j);
 
+/* link-compatible stubs for the reference's GPU gcd and division over a prime
+ * field (include/cumodp.h:105-106, src/cudaGcd.cu:401, src/cudaDiv.cu:294),
+ * which modpn's solver links (modpn/src/solve2.c:305,854,1049).  Outside
+ * this library's scope: they leave every output untouched, print one line to
+ * stderr and return -1.0f (the reference returns the elapsed time, >= 0). */
+CUMODP_API float gcdPrimeField(sfixn *Dgcd, sfixn *A, sfixn *B, sfixn *G, sfixn n, sfixn m, sfixn p,
+                               sfixn optionNormalize);
+CUMODP_API float divPrimeField(sfixn *A, sfixn *B, sfixn *R, sfixn *Q, sfixn n, sfixn m, sfixn p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,6 +155,12 @@ CUMODP_API cumodp_err cumodp_subproduct_tree(const sfixn *points, int k, sfixn p
  * tables cannot be allocated inside a capture: such a call then fails).  A
  * replay reads the device buffers named at capture time and the descriptors
  * as they were then; the graph owns the pinned copies of those descriptors.
+ * The graph also holds the library's scratch buffers of that stream (and of
+ * its auxiliary streams) until cumodp_graph_destroy: a later eager call on
+ * the stream that needs more scratch gets a new buffer, and destroying the
+ * stream leaves the held ones alone, so a replay never reads freed memory.
+ * A replay still shares that scratch with eager calls on the same stream,
+ * so run them in stream order (one stream), not concurrently.
  * No memset nodes are recorded (the library clears memory with kernels). */
 CUMODP_API cumodp_err cumodp_graph_begin(void *stream);
 CUMODP_API void *cumodp_graph_end(void *stream);

@@ -144,3 +144,22 @@ def test_two_convolution_rejects_narrow_output(cm):
     rc = cm.lib().cumodp_two_convolution_poly_mul(cm._ptr(A), 2, 1, cm._ptr(A), 2, 1,
                                                   cm._ptr(out), 1)
     assert rc == cm.CUMODP_ASSUMPTION_ERROR
+
+
+def test_gcd_and_division_link_stubs(cm):
+    """gcdPrimeField / divPrimeField (include/cumodp.h:105-106) resolve for
+    modpn's solver and report "not provided" without touching any output."""
+    L = cm.lib()
+    L.divPrimeField.restype = ctypes.c_float
+    L.gcdPrimeField.restype = ctypes.c_float
+    A = np.arange(1, 9, dtype=np.int32)
+    B = np.array([3, 1], np.int32)
+    R = np.full(1, -7, np.int32)
+    Q = np.full(7, -7, np.int32)
+    assert L.divPrimeField(cm._ptr(A), cm._ptr(B), cm._ptr(R), cm._ptr(Q), 8, 2, 469762049) == -1.0
+    assert (R == -7).all() and (Q == -7).all()
+    G = np.full(2, -7, np.int32)
+    d = ctypes.c_int(-7)
+    assert L.gcdPrimeField(ctypes.byref(d), cm._ptr(A), cm._ptr(B), cm._ptr(G), 8, 2, 469762049,
+                           1) == -1.0
+    assert d.value == -7 and (G == -7).all()

@@ -100,6 +100,50 @@ def test_graph_fast_evaluation(cm, orc):
         L.cumodp_stream_destroy(C.c_void_p(s))
 
 
+def test_graph_keeps_its_scratch(cm, orc):
+    """A graph holds the scratch it was captured with: an eager call on the
+    same stream that needs more scratch, and then the destruction of that
+    stream, must leave the replays correct (they would read freed memory
+    if the graph did not hold its buffers)."""
+    L = cm.lib()
+    small, big = cm.workload2(logn=15), cm.workload2(logn=20)
+    dA, dB = cm.DeviceArray.from_host(small["F"]), cm.DeviceArray.from_host(small["G"])
+    dH = cm.DeviceArray(2 * len(small["F"]) - 1)
+    dX, dY = cm.DeviceArray.from_host(big["F"]), cm.DeviceArray.from_host(big["G"])
+    dZ = cm.DeviceArray(2 * len(big["F"]) - 1)
+    want = orc.ntt_polymul(small["F"], small["G"], small["p"])
+    s = L.cumodp_stream_create()
+    s2 = L.cumodp_stream_create()
+    g = None
+    try:
+        run = lambda: cm.fftmul_uni_dev(dH.n - 1, dH, len(small["F"]) - 1, dA,
+                                        len(small["G"]) - 1, dB, small["p"], s)
+        g = cm.Graph.capture(s, run)
+        # grows the stream's scratch past the captured size
+        cm.fftmul_uni_dev(dZ.n - 1, dZ, len(big["F"]) - 1, dX, len(big["G"]) - 1, dY,
+                          big["p"], s)
+        assert L.cumodp_stream_synchronize(C.c_void_p(s)) == 0
+        dH.upload(np.zeros(dH.n, np.int32))
+        g.launch(s)
+        assert L.cumodp_stream_synchronize(C.c_void_p(s)) == 0
+        assert np.array_equal(dH.download(), want)
+        assert L.cumodp_stream_destroy(C.c_void_p(s)) == 0
+        s = None
+        # the capture stream is gone; the graph replays on another stream
+        for _ in range(2):
+            dH.upload(np.zeros(dH.n, np.int32))
+            g.launch(s2)
+            assert L.cumodp_stream_synchronize(C.c_void_p(s2)) == 0
+            assert np.array_equal(dH.download(), want)
+        assert np.array_equal(dZ.download(), orc.ntt_polymul(big["F"], big["G"], big["p"]))
+    finally:
+        if g is not None:
+            g.destroy()
+        if s is not None:
+            L.cumodp_stream_destroy(C.c_void_p(s))
+        L.cumodp_stream_destroy(C.c_void_p(s2))
+
+
 def test_graph_requires_created_stream(cm):
     assert cm.lib().cumodp_graph_begin(None) == cm.CUMODP_ASSUMPTION_ERROR
 

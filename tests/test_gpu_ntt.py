@@ -319,6 +319,32 @@ def test_bivariate_transforms(cm, orc):
         assert (Y == X).all(), (em, en)
 
 
+def test_list_bivariate_transforms(cm, orc):
+    """list_bivariate_stockham_dev (include/list_stockham.h:94-95): q arrays
+    of 2^em x 2^en, each equal to the bivariate DFT of that array, through
+    both overloads (power tables and roots), with em != en and em = 1."""
+    rng = np.random.default_rng(18)
+    for p in (P1, P2):
+        for q, em, en in [(3, 4, 9), (2, 9, 5), (5, 1, 6), (1, 6, 14), (7, 3, 4)]:
+            wm, wn = orc.primitive_root(em, p), orc.primitive_root(en, p)
+            m, n = 1 << em, 1 << en
+            X = rand_poly(rng, q * m * n, p)
+            want = np.concatenate([biv_oracle(orc, X[b * m * n:(b + 1) * m * n], em, wm, en, wn,
+                                              p) for b in range(q)])
+            d = cm.DeviceArray.from_host(X)
+            cm.cxx("list_bivariate_stockham_dev_w")(C.c_void_p(d.ptr), q, em, wm, en, wn, p)
+            assert (d.download() == want).all(), (p, q, em, en, "roots")
+            dWm = cm.DeviceArray.from_host(orc.get_powers(max(m // 2, 1), wm, p))
+            dWn = cm.DeviceArray.from_host(orc.get_powers(n // 2, wn, p))
+            d2 = cm.DeviceArray.from_host(X)
+            cm.cxx("list_bivariate_stockham_dev_W")(C.c_void_p(d2.ptr), q, em,
+                                                    C.c_void_p(dWm.ptr), en,
+                                                    C.c_void_p(dWn.ptr), p)
+            assert (d2.download() == want).all(), (p, q, em, en, "tables")
+            for a in (d, d2, dWm, dWn):
+                a.free()
+
+
 def test_bivariate_multiply(cm, orc):
     """bi_stockham_poly_mul_dev is the 2-D cyclic convolution."""
     rng = np.random.default_rng(9)

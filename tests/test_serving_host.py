@@ -28,6 +28,8 @@ def _server(cap):
     s = object.__new__(serving.ZmulServer)
     s.slot_bytes = cap
     s._plans = {}
+    s.max_batch_s = None
+    s.sec_per_unit = None
     # scratch estimate without the library: 2 x 8 x Ly words per prime, 2 primes
     s._scratch_bytes = lambda bits, ny: 4 * 2 * 8 * (1 << max(4, (int(ny) - 1).bit_length())) * 2
     return s
@@ -46,6 +48,26 @@ def test_admission_respects_the_footprint_cap():
     # only arrived requests are considered
     s = _server(100 * one)
     assert s._fit(Q, 2, 5) == 5
+
+
+def test_admission_respects_the_service_time_cap():
+    """Once a rate is learnt, a batch stops at the estimated time cap; the
+    first request is always admitted; before any batch is measured only the
+    footprint cap applies."""
+    Q = _fake_requests([100] * 10, [100] * 10, [64] * 10)
+    s = _server(1 << 40)
+    s.max_batch_s = 1e-3
+    assert s._fit(Q, 0, 10) == 10           # no rate yet
+    u = s._units(64, 199)
+    s._learn(4 * u, 4e-3)                   # 1 ms per request
+    assert abs(s.sec_per_unit * u - 1e-3) < 1e-12
+    assert s._fit(Q, 0, 10) == 1
+    s.max_batch_s = 3.5e-3
+    assert s._fit(Q, 0, 10) == 3
+    assert s._fit(Q, 8, 10) == 10
+    # the rate is an exponentially weighted average of the batches served
+    s._learn(u, 2e-3)
+    assert abs(s.sec_per_unit * u - (0.8 * 1e-3 + 0.2 * 2e-3)) < 1e-12
 
 
 def test_latency_summary():
