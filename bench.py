@@ -81,6 +81,50 @@ def cpu_baseline(F, G, p, reps=2):
 
 
 RAMP_S = 0.25
+VALU_PEAK = 1024 * 2.32e9 / 4    # wave-instructions/s: 1024 SIMDs, 4 cycles each, 2.32 GHz
+CPU_THREADS = 16                 # the GPU box's host-core share for the OpenMP CPU legs
+
+
+def source_hash():
+    """Hash of the library's kernel and host sources: counter profiles record
+    the hash they were taken with, so counters from other sources show."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "cumodp_amd", "csrc")
+    for f in sorted(os.listdir(d)):
+        if f.endswith((".hip", ".h", ".cpp")):
+            h.update(f.encode())
+            with open(os.path.join(d, f), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def load_profile(name):
+    """A committed counter summary (profiles/<name>) and whether it was
+    taken from the current sources."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, False
+    with open(path) as f:
+        prof = json.load(f)
+    return prof, prof.get("sources_sha16") == source_hash()
+
+
+def valu_issue(line, seconds):
+    """VALU issue fraction of one call of a secondary line: its SQ_INSTS_VALU
+    per call (rocprofv3, profiles/valu_secondary.json) over the live time of
+    the call, against the issue peak; None without a current profile."""
+    prof, current = load_profile("valu_secondary.json")
+    if not prof or line not in prof.get("lines", {}):
+        return None
+    insts = prof["lines"][line]["valu_insts_per_call"]
+    out = {"insts_per_call": insts, "achieved": insts / seconds / 1e9,
+           "peak": VALU_PEAK / 1e9, "unit": "G wave-instructions/s",
+           "frac": insts / seconds / VALU_PEAK, "profile": "profiles/valu_secondary.json",
+           "profile_from_current_sources": current}
+    if not current:
+        out["frac"] = None      # counters of other kernels: not reported as current
+    return out
 
 
 def ramp(fn, sync, seconds=RAMP_S):
@@ -127,6 +171,99 @@ def graph_replay(cm, fn_on, out, eager, reps):
     finally:
         L.cumodp_stream_destroy(C.c_void_p(st))
     return t, same
+
+
+def bench_multiply_2048(cm, cpu, reps=200):
+    """The 2048-point multiply mod 469762049 (two degree-1023 operands,
+    SplitMix64 seed 1): the latency of one call (device buffers; and the host
+    entry point cumodp_fftmul_uni with its copies), the latency of one forward
+    2^11-point transform (stockham_dev on a device buffer, synchronous, the
+    operation the reference publishes at 0.339872 ms on a Tesla M2050,
+    new_tests/BigPrimeFieldFFT_3/test/timingReports/FFT_timings_TeslaM2050_node10:45),
+    and the throughput of 4096 such products in one list call.  CPU legs: the
+    oracle's schoolbook and serial NTT multiply on the same pair (one thread),
+    and the 4096 products on one thread and on CPU_THREADS OpenMP threads."""
+    import ctypes as C
+    w = cm.workload1()
+    p, F, G = w["p"], w["F"], w["G"]
+    L = cm.lib()
+    nout = len(F) + len(G) - 1
+    dF, dG, dH = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(G), cm.DeviceArray(nout)
+    st = L.cumodp_stream_create()
+    sync = lambda: L.cumodp_stream_synchronize(C.c_void_p(st))
+    one = lambda: cm.fftmul_uni_dev(nout - 1, dH, len(F) - 1, dF, len(G) - 1, dG, p, st)
+    one()
+    t_dev = timed(one, reps, sync)
+    H = dH.download()
+    Hh = np.zeros(nout, np.int32)
+    host = lambda: cm._check("cumodp_fftmul_uni", cm.cumodp_fftmul_uni(nout - 1, Hh, len(F) - 1,
+                                                                       F, len(G) - 1, G, p))
+    host()
+    t_host = timed(host, reps, lambda: None)
+    # one forward 2^11 transform in place on a device buffer (synchronous)
+    X = np.concatenate([F, np.zeros(1024, np.int32)])
+    dX = cm.DeviceArray.from_host(X)
+    w11 = next(x for x in (pow(g, (p - 1) >> 11, p) for g in range(2, 64))
+               if pow(x, 1 << 10, p) != 1)          # a root of order exactly 2^11
+    fwd = cm.cxx("stockham_dev_w")
+    fwd.restype = None
+    t_fwd = timed(lambda: fwd(C.c_void_p(dX.ptr), 2048, 11, w11, p), reps, lambda: None)
+    Y = dX.download()
+    # 4096 independent degree-1023 pairs in one batched call
+    pairs = 4096
+    rng = cm.SplitMix64(101)
+    FF = cm.uniform_mod(rng, pairs * 1024, p)
+    GG = cm.uniform_mod(rng, pairs * 1024, p)
+    items, nf, ng, nh = cm.make_items(np.full(pairs, 1023), np.full(pairs, 1023))
+    dFF, dGG, dHH = cm.DeviceArray.from_host(FF), cm.DeviceArray.from_host(GG), cm.DeviceArray(nh)
+    batch = lambda: cm.list_fftmul_dev(items, dFF, dGG, dHH, p, st)
+    batch()
+    t_batch = timed(batch, 20, sync)
+    HH = dHH.download()
+    L.cumodp_stream_destroy(C.c_void_p(st))
+    for d in (dF, dG, dH, dX, dFF, dGG, dHH):
+        d.free()
+    out = {"workload": "two degree-1023 polynomials mod 469762049, NTT size 2048",
+           "latency_us_device": t_dev * 1e6, "latency_us_host_api_with_copies": t_host * 1e6,
+           "forward_ntt_2e11_us": t_fwd * 1e6,
+           "published_forward_ntt_2e11_us_tesla_m2050": 339.872,
+           "batched_4096_pairs": {"ms": t_batch * 1e3, "coeffs_per_s": nh / t_batch,
+                                  "products_per_s": pairs / t_batch},
+           "check_host_eq_device": bool(np.array_equal(H, Hh)),
+           "valu_issue_batched": valu_issue("multiply_2048", t_batch)}
+    if cpu:
+        import oracle
+        oracle.lib()
+        out["check_bit_exact_vs_oracle"] = bool(np.array_equal(H, oracle.plain_polymul(F, G, p)))
+        out["check_forward_ntt_vs_oracle"] = bool(np.array_equal(Y, oracle.stockham(X, w11, p)))
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle.plain_polymul(F, G, p)
+            ts.append(time.perf_counter() - t0)
+        t_school = float(np.median(ts))
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            oracle.ntt_polymul(F, G, p)
+            ts.append(time.perf_counter() - t0)
+        t_ntt = float(np.median(ts))
+        desc = np.array([[it.f_off, it.g_off, it.h_off, it.df, it.dg] for it in items], np.int64)
+        t0 = time.perf_counter()
+        H1 = oracle.polymul_items(desc, FF, GG, nh, p, plain_cut=0, threads=1)
+        t_1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Hn = oracle.polymul_items(desc, FF, GG, nh, p, plain_cut=0, threads=CPU_THREADS)
+        t_n = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": nh / t_1, "unit": "coeffs/s", "cores": 1, "kind": "port",
+            "sample": "the 4096 pairs, one serial oracle NTT multiply each",
+            "schoolbook_us_per_product": t_school * 1e6, "ntt_us_per_product": t_ntt * 1e6,
+            "openmp": {"threads": CPU_THREADS, "nproc": os.cpu_count(),
+                       "affinity_cores": len(os.sched_getaffinity(0)),
+                       "coeffs_per_s": nh / t_n},
+            "check_bit_exact": bool(np.array_equal(H1, HH)) and bool(np.array_equal(Hn, HH))}
+    return out
 
 
 def bench_list_multiply(cm, cpu, reps=5):
@@ -498,6 +635,7 @@ def copy_calibration(nbytes=256 << 20):
 
 
 SECONDARY = {
+    "multiply_2048": bench_multiply_2048,
     "list_multiply": bench_list_multiply,
     "plain_crossover": bench_plain_crossover,
     "subproduct_tree": bench_tree,

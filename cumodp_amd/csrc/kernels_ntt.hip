@@ -363,8 +363,9 @@ __attribute__((amdgpu_waves_per_eu(8, 8))) void k_col_fwd_d(
         cm_col_dif_first<TL, TOPZERO, true>(lds, X, n2, lim, lv, md);
     __syncthreads();
     u32 *const t[1] = {lds};
-    cm_dif_rounds<TL, 1, TL::NR - 1, 1>(t, lv, md);
-    cm_col_dif_last<TL>(lds, Y, n2, lv, md);
+    const MD mr = cm_rebind(md);
+    cm_dif_rounds<TL, 1, TL::NR - 1, 1>(t, lv, mr);
+    cm_col_dif_last<TL>(lds, Y, n2, lv, mr);
 }
 
 template <int LOGL1, int C, class MD>
@@ -468,11 +469,15 @@ __attribute__((amdgpu_waves_per_eu(LOGL2 == 12 ? 4 : 4, 8))) void k_row_mul(
         fa[m] = cm_buf_ld(rf, (u32)t << 2, (u32)(T * m) << 2);
         ga[m] = cm_buf_ld(rg, (u32)t << 2, (u32)(T * m) << 2);
     }
+    // one twiddle w_n^(k1 i) = rc b / 2^32 per element, shared by F and G; it
+    // stays in [0, 2p), which keeps each product below p 2^32 (an operand
+    // below 2p times a twiddle below 2p needs 4p < 2^32, p < 2^30, for the
+    // lazy form; canonical operands need 2p < 2^32)
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-        const u32 c = rc[k1 * 16 + m];
-        fa[m] = cm_mulw(cm_mont_lazy(fa[m], c, md.p, md.np), b, md);
-        ga[m] = cm_mulw(cm_mont_lazy(ga[m], c, md.p, md.np), b, md);
+        const u32 tw = cm_mont_lazy(rc[k1 * 16 + m], b, md.p, md.np);
+        fa[m] = cm_mulw(fa[m], tw, md);
+        ga[m] = cm_mulw(ga[m], tw, md);
     }
     // elements t + T m are thread t's group of the first forward and the last
     // inverse round: those rounds run on the registers
@@ -543,6 +548,7 @@ __global__ __launch_bounds__((Tile<LOGM, C, true>::T)) void k_col_ntt(
 #pragma unroll
     for (int r = 0; r < PER; ++r) lds[TL::phys(threadIdx.x + r * TL::T)] = xa[r];
     __syncthreads();
+    md = cm_rebind(md);
     cm_tile_dif<TL>(lds, lv, md);
     for (int idx = threadIdx.x; idx < TL::WORDS; idx += TL::T) {
         const int i = idx / C, c = idx % C;

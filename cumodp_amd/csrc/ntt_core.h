@@ -57,6 +57,20 @@ struct ModLazy {
     static constexpr bool LAZY = true;
 };
 
+// The same modulus as a value of this basic block.  Code that follows a
+// branch (the checked and unchecked first rounds) would otherwise take the
+// 64-bit zero extension of p from the entry block, whose zero high word the
+// instruction selector cannot see across blocks: it then expands each
+// m * p + t of a Montgomery product into two v_mad_u64_u32 and a v_mov_b32
+// instead of one v_mad_u64_u32.  readfirstlane of a uniform value costs no
+// vector instruction.
+template <class MD>
+CM_DEV MD cm_rebind(MD md) {
+    md.p = __builtin_amdgcn_readfirstlane(md.p);
+    md.np = __builtin_amdgcn_readfirstlane(md.np);
+    return md;
+}
+
 // the range bound c of the values between butterflies: p or 2p
 template <class MD>
 CM_DEV u32 cm_bound(MD md) { return MD::LAZY ? md.p << 1 : md.p; }
@@ -89,12 +103,33 @@ CM_DEV u32 cm_twiddle(const u32 *tab, int base_mm, int lane) {
 // (x, y) -> (x + y, (x - y) w) on values in [0, c): the difference enters
 // the product unreduced (x - y + c < 2c, and 2c w < p 2^32).  cm_reduce2p and
 // cm_sub are exact for any bound c with 2c < 2^32.
+//
+// Canonical residues (Mod, any p < 2^31) take the difference signed instead:
+// e = x - y lies in (-p, p) and t = e w in (-p^2, p^2).  With m = t (-p^-1)
+// mod 2^32 read as a signed 32-bit value, t + m p is divisible by 2^32, and
+// r = (t + m p) / 2^32 = e w / 2^32 mod p has |r| < (p^2 + 2^31 p) / 2^32 < p.
+// One conditional add of p (the smaller of r and r + p as unsigned words)
+// makes it canonical.  That is a subtraction, two v_mad_i64_i32, a
+// v_mul_lo_u32, an add and a min: one instruction fewer than adding p before
+// an unsigned Montgomery product and reducing after.
+CM_DEV u32 cm_mul_signed(u32 e, u32 w, u32 p, u32 np) {
+    const int64_t t = (int64_t)(int32_t)e * (int32_t)w;
+    const int32_t m = (int32_t)((u32)t * np);
+    const u32 r = (u32)(((int64_t)m * (int32_t)p + t) >> 32);
+    const u32 rp = r + p;
+    return rp < r ? rp : r;
+}
+
 template <class MD>
 CM_DEV void cm_dif_bfly(u32 &x, u32 &y, u32 w, MD md) {
     const u32 c = cm_bound(md);
     u32 s = cm_reduce2p(x + y, c);
-    u32 d = x - y + c;                 // (0, 2c)
-    y = cm_mulw(d, w, md);
+    if constexpr (MD::LAZY) {
+        u32 d = x - y + c;             // (0, 2c)
+        y = cm_mulw(d, w, md);
+    } else {
+        y = cm_mul_signed(x - y, w, md.p, md.np);
+    }
     x = s;
 }
 

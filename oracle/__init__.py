@@ -51,6 +51,7 @@ def lib():
         L.orc_list_stockham.argtypes = [i32p, s, s, s, s]
         L.orc_plain_polymul.argtypes = [s, i32p, s, i32p, s, i32p, s]
         L.orc_ntt_polymul.argtypes = [i32p, s, i32p, s, i32p, s]
+        L.orc_polymul_items.argtypes = [C.c_int64, i64p, i32p, i32p, i32p, s, s, s]
         L.orc_stockham_poly_mul_host.argtypes = [s, i32p, s, i32p, s, i32p, s]
         L.orc_stockham_poly_mul_host.restype = C.c_int
         L.orc_zmul_i64_schoolbook.argtypes = [i64p, C.c_int64, i64p, C.c_int64, u64p]
@@ -150,6 +151,16 @@ def poly_mul(F, G, p):
             or e > fourier_degree(p)):
         return plain_polymul(F, G, p)
     return ntt_polymul(F, G, p)
+
+
+def polymul_items(desc, F, G, nh, p, plain_cut=64, threads=1):
+    """Independent products (desc: (count, 5) int64 rows of f_off, g_off,
+    h_off, df, dg) on `threads` OpenMP threads; returns the nh-word H."""
+    desc = np.ascontiguousarray(desc, np.int64)
+    H = np.zeros(nh, np.int32)
+    lib().orc_polymul_items(desc.shape[0], desc.ravel(), _i32(F), _i32(G), H, p, plain_cut,
+                            threads)
+    return H
 
 
 def stockham_poly_mul_host(dh, H, F, G, p):

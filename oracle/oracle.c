@@ -274,6 +274,24 @@ void orc_ntt_polymul(sfixn *H, sfixn dF, const sfixn *F, sfixn dG,
     free(G0);
 }
 
+void orc_polymul_items(int64_t count, const int64_t *desc, const sfixn *F, const sfixn *G,
+                       sfixn *H, sfixn p, int plain_cut, int threads)
+{
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
+    for (int64_t r = 0; r < count; ++r) {
+        const int64_t *d = desc + 5 * r;
+        sfixn df = (sfixn)d[3], dg = (sfixn)d[4];
+        sfixn *h = H + d[2];
+        if ((df < dg ? df : dg) + 1 <= plain_cut) {
+            memset(h, 0, sizeof(sfixn) * (size_t)(df + dg + 1));
+            orc_plain_polymul(df + dg, h, df, F + d[0], dg, G + d[1], p);
+        } else {
+            orc_ntt_polymul(h, df, F + d[0], dg, G + d[1], p);
+        }
+    }
+}
+
 int orc_stockham_poly_mul_host(sfixn dh, sfixn *H, sfixn df, const sfixn *F,
                                sfixn dg, const sfixn *G, sfixn p)
 {

@@ -19,6 +19,7 @@
  *   orc_plain_polymul / orc_ntt_polymul  -- F=G=1 convolution-count closed form,
  *                                           equal to each other, and the
  *                                           reference's test_poly_mul vectors.
+ *   orc_polymul_items                    -- equal to those two per item.
  *   orc_zmul_*                           -- reference gtest KATs and the 27
  *                                           Maple a*b=c benchmark fixtures.
  *   orc_bit_width_mw, orc_limb_split,
@@ -81,6 +82,13 @@ void orc_plain_polymul(sfixn dH, sfixn *H, sfixn dF, const sfixn *F,
  * H gets dF+dG+1 coefficients */
 void orc_ntt_polymul(sfixn *H, sfixn dF, const sfixn *F, sfixn dG,
                      const sfixn *G, sfixn p);
+/* Independent products on up to `threads` OpenMP threads (the parallel leg of
+ * the CPU baseline): item r is desc[5r .. 5r+4] = (f_off, g_off, h_off, df,
+ * dg) and writes df+dg+1 words at H + h_off; the schoolbook when the shorter
+ * operand has at most `plain_cut` coefficients, else orc_ntt_polymul (the
+ * reference's own split: src/fft_aux.cu:805-820 below, :862-879 above). */
+void orc_polymul_items(int64_t count, const int64_t *desc, const sfixn *F, const sfixn *G,
+                       sfixn *H, sfixn p, int plain_cut, int threads);
 /* src/stockham.cu:562-611 host-wrapper semantics with the overrun fixed:
  * returns 0 and writes min(dh+1, n) words (product, then zeros), or returns
  * 1 and leaves H untouched when n < 256. */

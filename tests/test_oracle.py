@@ -317,3 +317,22 @@ def test_poly_eval_closed_forms(orc):
     # constant and zero polynomials
     assert (orc.poly_eval(np.array([5], np.int32), X, p) == 5).all()
     assert (orc.poly_eval(np.zeros(3, np.int32), X, p) == 0).all()
+
+
+def test_polymul_items_equals_per_item_products(orc):
+    """The OpenMP batch driver (the parallel CPU-baseline leg) gives each
+    item's schoolbook / NTT product, whatever the thread count."""
+    rng = np.random.default_rng(31)
+    p = 469762049
+    shapes = [(0, 0), (3, 900), (64, 64), (700, 5), (1500, 1400), (63, 2000)]
+    F = [rng.integers(0, p, a + 1) for a, _ in shapes]
+    G = [rng.integers(0, p, b + 1) for _, b in shapes]
+    desc, fo, go, ho = [], 0, 0, 0
+    for (a, b) in shapes:
+        desc.append([fo, go, ho, a, b])
+        fo, go, ho = fo + a + 1, go + b + 1, ho + a + b + 1
+    Fa, Ga = np.concatenate(F), np.concatenate(G)
+    for threads in (1, 4):
+        H = orc.polymul_items(np.array(desc), Fa, Ga, ho, p, plain_cut=64, threads=threads)
+        for (f0, g0, h0, a, b), f, g in zip(desc, F, G):
+            assert np.array_equal(H[h0:h0 + a + b + 1], orc.plain_polymul(f, g, p))
