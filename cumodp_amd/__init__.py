@@ -124,6 +124,8 @@ def lib():
         L.cumodp_last_error.restype = C.c_char_p
         L.cumodp_fftmul_uni_dev.argtypes = [s, vp, s, vp, s, vp, s, vp]
         L.cumodp_fftmul_uni_dev.restype = err
+        L.cumodp_fftmul_uni_dev_noalloc.argtypes = [s, vp, s, vp, s, vp, s, vp]
+        L.cumodp_fftmul_uni_dev_noalloc.restype = err
         L.cumodp_list_fftmul_dev.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp, s, vp]
         L.cumodp_list_fftmul_dev.restype = err
         L.cumodp_list_plainmul_dev.argtypes = [C.POINTER(MulItem), C.c_int64, vp, vp, vp, s, vp]
@@ -323,11 +325,13 @@ def list_stockham_host(X, m, k, w, p):
     return X
 
 
-def fftmul_uni_dev(dh, Hd, df, Fd, dg, Gd, p, stream=None):
-    """cumodp_fftmul_uni_dev on DeviceArray/int addresses (stream ordered)."""
+def fftmul_uni_dev(dh, Hd, df, Fd, dg, Gd, p, stream=None, noalloc=False):
+    """cumodp_fftmul_uni_dev on DeviceArray/int addresses (stream ordered);
+    noalloc=True calls cumodp_fftmul_uni_dev_noalloc (no device allocation,
+    transform size at most 2^22, tables built by an earlier allocating call)."""
     a = lambda x: x.addr() if isinstance(x, DeviceArray) else C.c_void_p(x)
-    _check("cumodp_fftmul_uni_dev", lib().cumodp_fftmul_uni_dev(
-        dh, a(Hd), df, a(Fd), dg, a(Gd), p, C.c_void_p(stream)))
+    fn = "cumodp_fftmul_uni_dev_noalloc" if noalloc else "cumodp_fftmul_uni_dev"
+    _check(fn, getattr(lib(), fn)(dh, a(Hd), df, a(Fd), dg, a(Gd), p, C.c_void_p(stream)))
 
 
 def list_fftmul_dev(items, Fd, Gd, Hd, p, stream=None, per_call=False):

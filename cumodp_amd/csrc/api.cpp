@@ -466,6 +466,53 @@ cumodp_err cumodp_fftmul_uni_dev(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd,
                         (u32)p, as_stream(stream));
 }
 
+// The same product with no allocation: twiddle tables are looked up, never
+// built, and the four-step work arrays are the file-scope device scratch of
+// noalloc.hip.  Per-kernel timing is suspended (it would create events).
+cumodp_err cumodp_fftmul_uni_dev_noalloc(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd,
+                                         sfixn dg, const sfixn *Gd, sfixn p, void *stream) {
+    if (df < 0 || dg < 0 || dh < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative degree");
+    const int64_t slen = (int64_t)df + dg + 1;
+    int e = ceil_log2(slen);
+    if (e < 4) e = 4;
+    if (e > kNoAllocMaxLog)
+        return fail(CUMODP_FFT_SIZE_TOO_LARGE,
+                    "non-allocating multiply: df + dg + 1 above 2^22 (its scratch holds two "
+                    "2^22-word arrays)");
+    cumodp_err ce = check_prime_size((u32)p, e);
+    if (ce != CUMODP_SUCCESS) return ce;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const Family *fam = family_default_cached((u32)p);
+    const Outer *o = (fam && e > kMaxLdsLog) ? outer_cached(fam, e) : nullptr;
+    if (!fam || (e > kMaxLdsLog && !o))
+        return fail(CUMODP_ASSUMPTION_ERROR,
+                    "non-allocating multiply: the twiddle tables of this prime and size are not "
+                    "built yet (one cumodp_fftmul_uni_dev call of the same size builds them)");
+    const int64_t n = (int64_t)1 << e;
+    const int64_t nout = (int64_t)dh + 1 < n ? (int64_t)dh + 1 : n;
+    hipStream_t s = as_stream(stream);
+    prof_suspend(true);
+    hipError_t he;
+    if (e <= kMaxLdsLog) {
+        MulDesc d;
+        d.f = 0; d.g = 0; d.h = 0;
+        d.lf = df + 1; d.lg = dg + 1; d.lh = (int)nout; d.logn = e;
+        he = launch_mul_lds_one(e, (const u32 *)Fd, (const u32 *)Gd, (u32 *)Hd, nullptr, 1, d,
+                                fam, s);
+    } else {
+        u32 *ws = nullptr;
+        he = noalloc_scratch(&ws);
+        if (he == hipSuccess) {
+            FourStepIO io{(const u32 *)Fd, (const u32 *)Gd, df + 1, dg + 1, 0, ws, ws + n,
+                          (u32 *)Hd, nout, 0, 1};
+            he = launch_mul_four_step_with(io, fam, o, s);
+        }
+    }
+    prof_suspend(false);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "non-allocating multiply", he);
+    return CUMODP_SUCCESS;
+}
+
 // ---- two-convolution integer multiply -------------------------------------
 static int tc_out_bits(int bits_a, int bits_b, int64_t na, int64_t nb) {
     int lg = 0;

@@ -39,6 +39,24 @@ CUMODP_API const char *cumodp_last_error(void);
  * ordered (returns before the product is complete). */
 CUMODP_API cumodp_err cumodp_fftmul_uni_dev(sfixn dh, sfixn *Hd, sfixn df, const sfixn *Fd, sfixn dg,
                                  const sfixn *Gd, sfixn p, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same product as cumodp_fftmul_uni_dev, bit for bit,
+ * with no hipMalloc, hipFree or other allocating call.
+ * - Limit: df + dg + 1 <= 2^22 (transform size at most 2^22); larger sizes
+ *   return CUMODP_FFT_SIZE_TOO_LARGE.
+ * - Scratch: a 32 MiB file-scope device array per GPU, held from the first
+ *   load of the library's code until the process ends.  Every call on one
+ *   GPU shares it and it keeps the last call's contents (nothing in it has
+ *   to start at zero).  Calls that overlap on one GPU (two streams, or two
+ *   threads) race on it: order them.
+ * - Twiddle tables are only looked up: the prime and transform size must
+ *   have been used once before by an allocating call (e.g.
+ *   cumodp_fftmul_uni_dev of the same size), else CUMODP_ASSUMPTION_ERROR.
+ * Stream ordered, like cumodp_fftmul_uni_dev. */
+CUMODP_API cumodp_err cumodp_fftmul_uni_dev_noalloc(sfixn dh, sfixn *Hd, sfixn df,
+                                                    const sfixn *Fd, sfixn dg, const sfixn *Gd,
+                                                    sfixn p, void *stream);
 
 /* ---- list multiply of pairs of uneven degree ------------------------------
  * One call multiplies `count` independent pairs mod p.  Item i reads

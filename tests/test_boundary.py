@@ -85,6 +85,18 @@ def test_fftmul_rejects_bad_primes_before_device(cm):
     assert (H == -1).all()
 
 
+def test_noalloc_multiply_rejects_before_device(cm):
+    """The non-allocating multiply checks its limit (transform size 2^22),
+    degrees and prime before any pointer is touched."""
+    f = cm.lib().cumodp_fftmul_uni_dev_noalloc
+    null = ctypes.c_void_p(0)
+    half = 1 << 21
+    assert f(2 * half, null, half, null, half, null, cm.P2, None) == cm.CUMODP_FFT_SIZE_TOO_LARGE
+    assert b"2^22" in cm.lib().cumodp_last_error()
+    assert f(10, null, -1, null, 5, null, cm.P2, None) == cm.CUMODP_ASSUMPTION_ERROR
+    assert f(10, null, 5, null, 5, null, 1000, None) == cm.CUMODP_ASSUMPTION_ERROR
+
+
 def test_oracle_is_not_linked_into_the_product(cm):
     r = subprocess.run(["ldd", cm.LIB_PATH], capture_output=True, text=True)
     assert "oracle" not in r.stdout
