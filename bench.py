@@ -154,11 +154,16 @@ def timed(fn, reps, sync):
     return float(np.median(ts))
 
 
+GRAPHS = True   # --no-graph: the lines skip their graph capture and replays
+
+
 def graph_replay(cm, fn_on, out, eager, reps):
     """Capture fn_on(stream) once on a created stream and time its replays
     (median of reps, after the clock ramp); returns (seconds, every replay
-    equal to the eager output)."""
+    equal to the eager output), or (None, None) under --no-graph."""
     import ctypes as C
+    if not GRAPHS:
+        return None, None
     L = cm.lib()
     st = L.cumodp_stream_create()
     try:
@@ -208,6 +213,8 @@ def bench_multiply_2048(cm, cpu, reps=200):
     fwd = cm.cxx("stockham_dev_w")
     fwd.restype = None
     t_fwd = timed(lambda: fwd(C.c_void_p(dX.ptr), 2048, 11, w11, p), reps, lambda: None)
+    dX.upload(X)               # the timed calls transformed it repeatedly in place
+    fwd(C.c_void_p(dX.ptr), 2048, 11, w11, p)
     Y = dX.download()
     # 4096 independent degree-1023 pairs in one batched call
     pairs = 4096
@@ -300,6 +307,7 @@ def bench_list_multiply(cm, cpu, reps=5):
            "per_call": {"ms": tc * 1e3, "coeffs_per_s": nh / tc},
            "graph_replay": {"ms": tg * 1e3, "coeffs_per_s": nh / tg},
            "batched_over_per_call": tc / tb,
+           "valu_issue_batched": valu_issue("list_multiply", tb),
            "check_batched_eq_per_call": bool(np.array_equal(H1, H2)),
            "check_graph_eq_batched": bool(np.array_equal(H1, H3)),
            "tuning": {"list_tile_words": L.cumodp_tune_get_list_tile(),
@@ -412,7 +420,8 @@ def bench_tree(cm, cpu, reps=5):
     ok = eval_mod(root, r, p) == _prod_mod([(r - int(x)) % p for x in pts], p)
     dP.free(), dT.free()
     out = {"workload": "subproduct tree, 2^16 points uniform mod 469762049, levels 0..16",
-           "ms": t * 1e3, "graph_replay_ms": tg * 1e3, "tree_words": nw,
+           "ms": t * 1e3, "graph_replay_ms": tg and tg * 1e3, "tree_words": nw,
+           "valu_issue": valu_issue("subproduct_tree", t),
            "check_root_evaluation": bool(ok), "check_graph_eq_eager": same_g}
     if cpu:
         import oracle
@@ -451,8 +460,9 @@ def bench_eval(cm, cpu, reps=5):
     for d in (dF, dP, dV):
         d.free()
     out = {"workload": "evaluate a degree-(2^16-1) polynomial at 2^16 points mod 469762049",
-           "ms": t * 1e3, "points_per_s": (1 << k) / t, "graph_replay_ms": tg * 1e3,
-           "graph_replay_points_per_s": (1 << k) / tg, "check_graph_eq_eager": same_g}
+           "ms": t * 1e3, "points_per_s": (1 << k) / t, "graph_replay_ms": tg and tg * 1e3,
+           "valu_issue": valu_issue("fast_evaluation", t),
+           "graph_replay_points_per_s": tg and (1 << k) / tg, "check_graph_eq_eager": same_g}
     if cpu:
         import oracle
         oracle.lib()
@@ -498,7 +508,8 @@ def bench_two_convolution(cm, cpu, reps=5):
     out = {"workload": "two degree-16383 int64 polynomials, coefficients uniform in "
                        "(-10^10, 10^10), exact product",
            "plan": cm.two_convolution_plan(bits, na, nb),
-           "ms": t * 1e3, "coeffs_per_s": nc / t}
+           "ms": t * 1e3, "coeffs_per_s": nc / t,
+           "valu_issue": valu_issue("two_convolution", t)}
     if cpu:
         import oracle
         oracle.lib()
@@ -554,13 +565,14 @@ def _eval_check(a, b, cw, x, q):
     return ev(_words_mod(cw, q)) == ev(am) * ev(bm) % q
 
 
-def bench_serving(cm, cpu, nreq=20000):
+def bench_serving(cm, cpu, nreq=20000, max_batch_ms=2.0):
     """Exact integer products served as they arrive: 20000 requests, degrees
     log-uniform in [64, 65536], coefficients of 16/32/48/64 bits (uniform
     choice), signed uniform.  First all requests arrive at once (capacity);
-    then Poisson arrivals at 80% of that capacity (exponential gaps from the
-    workload stream).  Latency = time from arrival to the product being back
-    in host memory."""
+    then three runs of Poisson arrivals at 80% of that capacity (exponential
+    gaps from the workload stream and from two more seeds), with batches
+    capped at max_batch_ms of estimated GPU work.  Latency = time from
+    arrival to the product being back in host memory."""
     from cumodp_amd import serving
     t0 = time.perf_counter()
     w = cm.workload5(requests=nreq)
@@ -568,18 +580,28 @@ def bench_serving(cm, cpu, nreq=20000):
     log(f"bench: serving: {Q.n} requests generated in {time.perf_counter() - t0:.1f} s, "
         f"{Q.total_coeffs} output coefficients")
     srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30)
+    runs = []
     try:
         srv.run(Q, np.zeros(Q.n))   # warm-up pass (plans, twiddles, buffers)
         done0, st0 = srv.run(Q, np.zeros(Q.n))
         cap_s = float(np.nanmax(done0))
         cap_rps = Q.n / cap_s
         lam = 0.8 * cap_rps
-        arr = np.cumsum(w["exp_gaps"][:Q.n]) / lam
-        done1, st1 = srv.run(Q, arr)
+        # three Poisson arrival streams at 80% of capacity (the workload's own
+        # gaps, then two more seeds); batches capped at max_batch_ms of
+        # estimated GPU work as well as by footprint
+        srv.max_batch_s = max_batch_ms * 1e-3
+        for seed, gaps in ((5, w["exp_gaps"]), (51, cm.exp_gaps(51, Q.n)),
+                           (52, cm.exp_gaps(52, Q.n))):
+            arr = np.cumsum(gaps[:Q.n]) / lam
+            done1, st1 = srv.run(Q, arr)
+            span = float(np.nanmax(done1) - arr[0])
+            runs.append({"gap_seed": seed, "coeffs_per_s": Q.total_coeffs / span,
+                         **serving.latency_summary(done1, arr), **st1})
     finally:
         srv.close()
-    span = float(np.nanmax(done1) - arr[0])
-    lat = serving.latency_summary(done1, arr)
+    lat = {"p50_ms": [r["p50_ms"] for r in runs], "p99_ms": [r["p99_ms"] for r in runs],
+           "p99_over_p50": [r["p99_ms"] / r["p50_ms"] for r in runs]}
     # size-independent check on a sample: evaluation at a random point mod
     # two 31-bit primes
     idx = np.linspace(0, Q.n - 1, 200).astype(int)
@@ -590,9 +612,10 @@ def bench_serving(cm, cpu, nreq=20000):
                        "measured capacity",
            "output_coeffs": Q.total_coeffs,
            "capacity": {"s": cap_s, "coeffs_per_s": Q.total_coeffs / cap_s,
-                        "requests_per_s": cap_rps, **st0},
-           "poisson": {"rate_requests_per_s": lam, "coeffs_per_s": Q.total_coeffs / span,
-                       **lat, **st1},
+                        "requests_per_s": cap_rps, "valu_issue": valu_issue("serving", cap_s),
+                        **st0},
+           "poisson": {"rate_requests_per_s": lam, "max_batch_ms": max_batch_ms, **lat,
+                       "runs": runs},
            "check_evaluation_sample": bool(ok)}
     if cpu:
         import oracle
@@ -634,6 +657,76 @@ def copy_calibration(nbytes=256 << 20):
     log(f"bench: copied {nbytes} bytes three times")
 
 
+def counter_calls(cm, line, ncalls):
+    """One warm-up call and `ncalls` more of a line's timed form (the batched
+    call; for serving the capacity run over every request), with nothing else
+    on the GPU.  A rocprofv3 --pmc run over this command gives the line's
+    counters per call: bench_counters.py sums them over the dispatches (the
+    runtime's own copy kernels aside) and divides by the calls printed here."""
+    import ctypes as C
+    L = cm.lib()
+    free = []
+    if line == "multiply_2048":
+        p, pairs = 469762049, 4096
+        rng = cm.SplitMix64(101)
+        FF, GG = cm.uniform_mod(rng, pairs * 1024, p), cm.uniform_mod(rng, pairs * 1024, p)
+        items, nf, ng, nh = cm.make_items(np.full(pairs, 1023), np.full(pairs, 1023))
+        dF, dG, dH = cm.DeviceArray.from_host(FF), cm.DeviceArray.from_host(GG), cm.DeviceArray(nh)
+        free += [dF, dG, dH]
+        run = lambda: cm.list_fftmul_dev(items, dF, dG, dH, p)
+    elif line == "list_multiply":
+        w = cm.workload4()
+        items, nf, ng, nh = cm.make_items(w["df"], w["dg"])
+        dF, dG, dH = (cm.DeviceArray.from_host(w["F"]), cm.DeviceArray.from_host(w["G"]),
+                      cm.DeviceArray(nh))
+        free += [dF, dG, dH]
+        run = lambda: cm.list_fftmul_dev(items, dF, dG, dH, w["p"])
+    elif line in ("subproduct_tree", "fast_evaluation"):
+        w = cm.workload4_tree()
+        p, pts, k = w["p"], w["points"], 16
+        dP = cm.DeviceArray.from_host(pts)
+        free.append(dP)
+        if line == "subproduct_tree":
+            dT = cm.DeviceArray(L.cumodp_subproduct_tree_words(k))
+            free.append(dT)
+            run = lambda: cm._check("cumodp_subproduct_tree_dev", L.cumodp_subproduct_tree_dev(
+                dP.addr(), k, p, dT.addr(), None))
+        else:
+            F = cm.uniform_mod(cm.SplitMix64(616), 1 << k, p)
+            dF, dV = cm.DeviceArray.from_host(F), cm.DeviceArray(1 << k)
+            free += [dF, dV]
+            run = lambda: cm._check("cumodp_fast_evaluation_dev", L.cumodp_fast_evaluation_dev(
+                dF.addr(), dP.addr(), k, p, dV.addr(), None))
+    elif line == "two_convolution":
+        w = cm.workload3()
+        A, B = cm._words(w["a"], 1), cm._words(w["b"], 1)
+        na, nb = len(w["a"]), len(w["b"])
+        dA, dB = cm.DeviceArray.from_host(A.ravel()), cm.DeviceArray.from_host(B.ravel())
+        dC = cm.DeviceArray((na + nb - 1) * 2, np.uint64)
+        free += [dA, dB, dC]
+        bits = L.cumodp_find_largest_bit_width(cm._ptr(A), na, 1, cm._ptr(B), nb, 1)
+        run = lambda: cm._check("cumodp_two_convolution_poly_mul_dev",
+                                L.cumodp_two_convolution_poly_mul_dev(
+                                    dA.addr(), na, 1, dB.addr(), nb, 1, dC.addr(), 2, bits, None))
+    elif line == "serving":
+        from cumodp_amd import serving
+        Q = serving.make_requests(cm.workload5())
+        srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30)
+        run = lambda: srv.run(Q, np.zeros(Q.n))
+    else:
+        raise SystemExit(f"no counter form for line {line}")
+    for i in range(1 + ncalls):
+        run()
+        cm.synchronize()
+        log(f"bench: counter call {i + 1} of {1 + ncalls} ({line}) done")
+    if line == "serving":
+        srv.close()
+        Q.free()
+    for d in free:
+        d.free()
+    return {"line": line, "calls": 1 + ncalls, "sources_sha16": source_hash()}
+
+
 SECONDARY = {
     "multiply_2048": bench_multiply_2048,
     "list_multiply": bench_list_multiply,
@@ -657,11 +750,18 @@ def main():
                          "list tile) before the timed runs")
     ap.add_argument("--serving-requests", type=int, default=20000,
                     help="requests of the serving line (0 skips it)")
+    ap.add_argument("--serving-max-batch-ms", type=float, default=2.0,
+                    help="service-time cap of a serving batch under Poisson arrivals")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (reference for "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--only", choices=sorted(SECONDARY) + ["serving"],
                     help="run one secondary line alone and print it (for profiling)")
+    ap.add_argument("--counter-calls", type=int, default=0,
+                    help="with --only: run only 1 + N calls of the line's timed form and print "
+                         "the call count (the command profiled for counters per call)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="with --only: skip the graph capture and replays of the line")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -670,11 +770,17 @@ def main():
         print(f"bench.py measures one GPU only (the path is single-GPU); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if args.only and args.counter_calls > 0:
+        import cumodp_amd as cm
+        print(json.dumps(counter_calls(cm, args.only, args.counter_calls)), flush=True)
+        return
     if args.only:
         import cumodp_amd as cm
+        global GRAPHS
+        GRAPHS = not args.no_graph
         cpu = not args.no_cpu_baseline
         if args.only == "serving":
-            r = bench_serving(cm, cpu, args.serving_requests)
+            r = bench_serving(cm, cpu, args.serving_requests, args.serving_max_batch_ms)
         else:
             r = SECONDARY[args.only](cm, cpu)
         print(json.dumps({args.only: r}), flush=True)
@@ -753,15 +859,19 @@ def main():
     # (calibrated on a copy kernel), kept in profiles/
     # and the VALU instructions per launch (SQ_INSTS_VALU): the issue rate,
     # not HBM, bounds these kernels
+    # (reported only when the profile was taken from the current kernel
+    # sources; a stale one is named with its figures set aside)
     traffic, counted = None, {}
-    valu_peak = 1024 * 2.32e9 / 4          # wave-instructions/s: 4 cycles each per SIMD
-    tpath = os.path.join(ROOT, "profiles", "traffic_2e24.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
+    valu_peak = VALU_PEAK
+    tj, current = load_profile("traffic_2e24.json")
+    counter_src = {"profile": "profiles/traffic_2e24.json", "from_current_sources": current,
+                   "sources_sha16": source_hash()}
+    if tj and current:
         counted = tj["kernels"]
         valu_peak = tj.get("valu_peak_per_s", valu_peak)
         traffic = counted.get(name, {}).get("traffic_bytes")
+    elif tj:
+        counter_src["stale_traffic_bytes"] = tj["kernels"].get(name, {}).get("traffic_bytes")
     kernels = {}
     for k, v in prof.items():
         a = v[0] / v[1] * 1e-3
@@ -795,7 +905,7 @@ def main():
                    "n": n, "p": p},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algo_bytes_per_launch": algo,
+                     "traffic": traffic, "counters": counter_src, "algo_bytes_per_launch": algo,
                      "avg_launch_us": avg_s * 1e6, "valu_issue": valu},
         "whole_multiply_hbm_frac_48n": whole / HBM_PEAK_GBS,
         "kernels": kernels,
@@ -842,7 +952,8 @@ def main():
             log(f"bench: {key}: {sec[key]}")
         if args.serving_requests > 0:
             log("bench: serving")
-            sec["serving"] = bench_serving(cm, cpu, args.serving_requests)
+            sec["serving"] = bench_serving(cm, cpu, args.serving_requests,
+                                           args.serving_max_batch_ms)
             log(f"bench: serving: {sec['serving']}")
         out["secondary"] = sec
     print(json.dumps(out), flush=True)

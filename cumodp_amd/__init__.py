@@ -63,7 +63,18 @@ CXX_SYMBOLS = {
     "list_bivariate_stockham_dev_W": "_Z27list_bivariate_stockham_devPiiiPKiiS1_i",
     "list_bivariate_stockham_dev_w": "_Z27list_bivariate_stockham_devPiiiiiii",
     "mulCUDA": "_Z7mulCUDAPiS_S_iiii",
+    "fastEvaluation": "_Z14fastEvaluationPiS_iii",
+    "onlySubproductTree": "_Z18onlySubproductTreePiS_iii",
+    "onlyTrees": "_Z9onlyTreesPiiii",
 }
+
+MAX_LEVEL = 27
+
+
+class PolyEvalSteps(C.Structure):
+    """struct PolyEvalSteps of include/fastPolyEvaluation.h (device pointers)"""
+    _fields_ = [(f, C.c_void_p * MAX_LEVEL) for f in ("Ml", "Mr", "InvMl", "InvMr", "fL", "fR")] + \
+               [("timeSubTree", C.c_float), ("timeSinvTree", C.c_float), ("timeEva", C.c_float)]
 
 
 class CumodpError(RuntimeError):
@@ -156,6 +167,9 @@ def lib():
         L.cumodp_tune_set_list_plain.argtypes = [s]
         L.cumodp_tune_set_list_plain.restype = err
         L.cumodp_tune_get_list_plain.restype = s
+        L.cumodp_tune_set_list_persistent.argtypes = [s]
+        L.cumodp_tune_set_list_persistent.restype = err
+        L.cumodp_tune_get_list_persistent.restype = s
         L.cumodp_tune_set_zmul_merge.argtypes = [s]
         L.cumodp_tune_set_zmul_merge.restype = err
         L.cumodp_tune_get_zmul_merge.restype = s
@@ -463,6 +477,47 @@ def fast_evaluation(F, points, p):
     _check("cumodp_fast_evaluation", lib().cumodp_fast_evaluation(_ptr(Fw), _ptr(pts), k, p,
                                                                   _ptr(out)))
     return out
+
+
+def eval_steps(fn, points, p, F=None, flag=0):
+    """Calls the reference-named driver fn ("fastEvaluation", "onlySubproductTree"
+    or "onlyTrees") and returns its PolyEvalSteps as host arrays: a dict
+    field -> {level: int32 array}, every device array freed afterwards.
+    None when the call failed (all pointers null)."""
+    pts = _i32(points)
+    k = len(pts).bit_length() - 1
+    assert len(pts) == 1 << k
+    f = cxx(fn)
+    f.restype = PolyEvalSteps
+    if fn == "onlyTrees":
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        r = f(_ptr(pts), k, p, flag)
+    else:
+        Fw = np.zeros(1 << k, np.int32)
+        if F is not None:
+            Fw[:len(F)] = _i32(F)
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+        r = f(_ptr(Fw), _ptr(pts), k, p, flag)
+    nh = (1 << k) // 2
+    out, any_set = {}, False
+    for name in ("Ml", "Mr", "InvMl", "InvMr", "fL", "fR"):
+        arr = getattr(r, name)
+        levels = {}
+        for i in range(MAX_LEVEL):
+            if not arr[i]:
+                continue
+            any_set = True
+            if name in ("Ml", "Mr"):
+                words = (nh >> i) * ((1 << i) + 1 if i < 8 else 1 << i)
+            else:
+                words = nh
+            h = np.empty(words, np.int32)
+            _check("cumodp_memcpy_d2h", lib().cumodp_memcpy_d2h(_ptr(h), C.c_void_p(arr[i]),
+                                                               h.nbytes))
+            lib().cumodp_dev_free(C.c_void_p(arr[i]))
+            levels[i] = h
+        out[name] = levels
+    return out if any_set else None
 
 
 # ---- graphs of repeated shapes -------------------------------------------------

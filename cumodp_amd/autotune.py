@@ -15,6 +15,9 @@ There are four tunables:
   * The plain cut-off of the list multiply: products whose shorter operand
     has at most this many coefficients run on the schoolbook kernel, the
     rest on the NTT. `cumodp_tune_set_list_plain`.
+  * The list multiply's launch form: one persistent launch for every
+    product up to 2^15 points, or one launch per size class (the mixed-size
+    tile above applies to the latter). `cumodp_tune_set_list_persistent`.
 
 Each candidate is timed on the operation it affects. The fastest candidate
 is applied to the library, and the choices can be saved to and loaded from
@@ -106,6 +109,29 @@ def tune_list_tile(reps=5, log=None, pairs=4096):
     return b
 
 
+def tune_list_persistent(reps=5, log=None, pairs=4096):
+    """Time the uneven list multiply (the 4096-pair workload) in its
+    persistent and its per-class launch form; apply and return the best."""
+    L = lib()
+    w = workload4(pairs=pairs)
+    items, nf, ng, nh = make_items(w["df"], w["dg"])
+    dF, dG, dH = DeviceArray.from_host(w["F"]), DeviceArray.from_host(w["G"]), DeviceArray(nh)
+    fn = lambda: list_fftmul_dev(items, dF, dG, dH, w["p"])
+    times = {}
+    for on in (1, 0):
+        _check("cumodp_tune_set_list_persistent", L.cumodp_tune_set_list_persistent(on))
+        times[on] = _median_time(fn, reps)
+    b = min(times, key=times.get)
+    _check("cumodp_tune_set_list_persistent", L.cumodp_tune_set_list_persistent(b))
+    if log:
+        log("autotune: list multiply launch: " +
+            ", ".join(f"{'persistent' if c else 'per class'}: {t * 1e3:.3f} ms"
+                      for c, t in times.items()) + f" -> {'persistent' if b else 'per class'}")
+    for d in (dF, dG, dH):
+        d.free()
+    return b
+
+
 def tune_list_plain(reps=5, log=None, pairs=4096, cutoffs=(0, 64, 128, 192, 256, 384)):
     """Time the uneven list multiply (the 4096-pair workload) for each plain
     cut-off; apply and return the best."""
@@ -153,10 +179,17 @@ def tune_col_tile(k=24, reps=5, p=P2, log=None):
 
 
 def tune_all(log=None, ks=range(14, 25)):
+    L = lib()
     col = tune_col_tile(log=log)
-    return {"col_tile_words": col, "split": tune_split(ks=ks, log=log),
-            "list_tile_words": tune_list_tile(log=log),
-            "list_plain_cutoff": tune_list_plain(log=log)}
+    split = tune_split(ks=ks, log=log)
+    # the mixed-size tile matters to the per-class form only
+    on = L.cumodp_tune_get_list_persistent()
+    _check("cumodp_tune_set_list_persistent", L.cumodp_tune_set_list_persistent(0))
+    tile = tune_list_tile(log=log)
+    _check("cumodp_tune_set_list_persistent", L.cumodp_tune_set_list_persistent(on))
+    persistent = tune_list_persistent(log=log)
+    return {"col_tile_words": col, "split": split, "list_tile_words": tile,
+            "list_persistent": persistent, "list_plain_cutoff": tune_list_plain(log=log)}
 
 
 def current():
@@ -164,7 +197,8 @@ def current():
     return {"split": {k: L.cumodp_tune_get_split(k) for k in range(14, 27)},
             "list_tile_words": L.cumodp_tune_get_list_tile(),
             "col_tile_words": L.cumodp_tune_get_col_tile(),
-            "list_plain_cutoff": L.cumodp_tune_get_list_plain()}
+            "list_plain_cutoff": L.cumodp_tune_get_list_plain(),
+            "list_persistent": L.cumodp_tune_get_list_persistent()}
 
 
 def apply(choices):
@@ -180,6 +214,9 @@ def apply(choices):
     if "list_plain_cutoff" in choices:
         _check("cumodp_tune_set_list_plain",
                L.cumodp_tune_set_list_plain(int(choices["list_plain_cutoff"])))
+    if "list_persistent" in choices:
+        _check("cumodp_tune_set_list_persistent",
+               L.cumodp_tune_set_list_persistent(int(choices["list_persistent"])))
 
 
 def save(choices, path):
@@ -187,7 +224,8 @@ def save(choices, path):
         json.dump({"split": {str(k): v for k, v in choices["split"].items()},
                    "list_tile_words": choices["list_tile_words"],
                    "col_tile_words": choices.get("col_tile_words", 16384),
-                   "list_plain_cutoff": choices.get("list_plain_cutoff", 256)}, f, indent=1)
+                   "list_plain_cutoff": choices.get("list_plain_cutoff", 256),
+                   "list_persistent": choices.get("list_persistent", 1)}, f, indent=1)
 
 
 def load(path):

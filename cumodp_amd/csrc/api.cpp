@@ -12,6 +12,7 @@
 
 #include "../../include/cumodp.h"
 #include "../../include/cumodp_amd.h"
+#include "../../include/fastPolyEvaluation.h"
 #include "../../include/fft_aux.h"
 #include "../../include/list_inv_stockham.h"
 #include "../../include/list_stockham.h"
@@ -847,6 +848,14 @@ cumodp_err cumodp_tune_set_list_plain(int cutoff) {
 
 int cumodp_tune_get_list_plain(void) { return list_plain_cutoff(); }
 
+cumodp_err cumodp_tune_set_list_persistent(int on) {
+    std::lock_guard<std::mutex> lk(api_mutex());
+    return set_list_persistent(on) ? CUMODP_SUCCESS
+                                   : fail(CUMODP_ASSUMPTION_ERROR, "list persistent: 0 or 1");
+}
+
+int cumodp_tune_get_list_persistent(void) { return list_persistent(); }
+
 cumodp_err cumodp_tune_set_col_tile(int words) {
     std::lock_guard<std::mutex> lk(api_mutex());
     return set_col_tile_words(words) ? CUMODP_SUCCESS
@@ -931,6 +940,131 @@ cumodp_err cumodp_list_fftmul_dev_percall(const cumodp_mul_item *items, int64_t 
 // ===========================================================================
 // C++ linkage drop-ins
 // ===========================================================================
+
+// ---- the reference's evaluation drivers (include/fastPolyEvaluation.h) ----
+// One fast-evaluation run (tree, subinverse tree, remainder tree) in the
+// library's layout, then device copies into separately allocated arrays in
+// the reference's half-tree layout (src/fastPolyEvaluation.cu:512-656,
+// :1371-1470).
+namespace {
+
+enum class StepsWanted { kTree, kTrees, kEval };
+
+// a new allocation holding `rows` rows of `width` words read `pitch` words
+// apart from src (the caller of the drop-in frees it)
+hipError_t steps_copy(sfixn **dst, const u32 *src, int64_t rows, int64_t width, int64_t pitch) {
+    void *d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(u32) * (size_t)(rows * width));
+    if (e != hipSuccess) return e;
+    *dst = (sfixn *)d;
+    if (pitch == width || rows == 1)
+        return hipMemcpyAsync(d, src, sizeof(u32) * (size_t)(rows * width),
+                              hipMemcpyDeviceToDevice, 0);
+    return hipMemcpy2DAsync(d, sizeof(u32) * (size_t)width, src, sizeof(u32) * (size_t)pitch,
+                            sizeof(u32) * (size_t)width, (size_t)rows, hipMemcpyDeviceToDevice,
+                            0);
+}
+
+void steps_free(PolyEvalSteps &r) {
+    for (int i = 0; i < MAX_LEVEL; ++i) {
+        for (sfixn *q : {r.Ml[i], r.Mr[i], r.InvMl[i], r.InvMr[i], r.fL[i], r.fR[i]})
+            if (q) (void)hipFree(q);
+    }
+    memset(&r, 0, sizeof(r));
+}
+
+cumodp_err steps_run(PolyEvalSteps &r, const sfixn *F, const sfixn *points, int k, sfixn p,
+                     StepsWanted what, bool keep_all) {
+    if (k < 1 || k >= MAX_LEVEL) return fail(CUMODP_ASSUMPTION_ERROR, "k must be in 1 .. 26");
+    cumodp_err ce = check_prime_size((u32)p, k < 4 ? 4 : k);
+    if (ce != CUMODP_SUCCESS) return ce;
+    std::lock_guard<std::mutex> lk(api_mutex());
+    const Family *fam = family_default((u32)p);
+    if (!fam) return fail(CUMODP_OUT_OF_MEMORY, "twiddle tables");
+    const int64_t n = (int64_t)1 << k, nh = n / 2;
+    const bool eval = what == StepsWanted::kEval;
+    u32 *io = (u32 *)workspace(19, sizeof(u32) * (size_t)(3 * n), 0);
+    u32 *scr = (u32 *)workspace(15, sizeof(u32) * (size_t)eval_scratch_words(k), 0);
+    u32 *kept = keep_all ? (u32 *)workspace(18, sizeof(u32) * (size_t)(k * n), 0) : nullptr;
+    if (!io || !scr || (keep_all && !kept)) return fail(CUMODP_OUT_OF_MEMORY, "scratch");
+    u32 *dpts = io, *dF = io + n, *dval = io + 2 * n;
+    hipError_t he = hipMemcpy(dpts, points, sizeof(u32) * (size_t)n, hipMemcpyHostToDevice);
+    if (he == hipSuccess && eval)
+        he = hipMemcpy(dF, F, sizeof(u32) * (size_t)n, hipMemcpyHostToDevice);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "copy in", he);
+    EvalKeep keep;
+    keep.trees_only = !eval;
+    keep.rem = kept;
+    he = launch_fast_evaluation(dF, dpts, k, (u32)p, dval, scr, fam, 0, &keep);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "fast evaluation", he);
+    const bool trees = !eval || keep_all;
+    if (trees) {
+        he = hipMalloc((void **)&r.Ml[0], sizeof(u32) * (size_t)(2 * nh));
+        if (he == hipSuccess) he = hipMalloc((void **)&r.Mr[0], sizeof(u32) * (size_t)(2 * nh));
+        if (he == hipSuccess)
+            he = launch_ref_leaves(dpts, nh, (u32 *)r.Ml[0], (u32 *)r.Mr[0], 0);
+        for (int i = 1; i < k && he == hipSuccess; ++i) {
+            const int64_t nodes = nh >> i, len = ((int64_t)1 << i) + 1;
+            const u32 *lv = scr + tree_level_offset(k, i);
+            if (i < plainMulLimit) {
+                he = steps_copy(&r.Ml[i], lv, 1, nodes * len, nodes * len);
+                if (he == hipSuccess)
+                    he = steps_copy(&r.Mr[i], lv + nodes * len, 1, nodes * len, nodes * len);
+            } else {
+                he = steps_copy(&r.Ml[i], lv, nodes, len - 1, len);
+                if (he == hipSuccess) he = steps_copy(&r.Mr[i], lv + nodes * len, nodes, len - 1, len);
+            }
+        }
+        // subinverse levels: plainMulLimit only for the tree alone, else
+        // plainMulLimit .. k-1 (the library keeps levels min(k,4) .. k-1)
+        const int top = what == StepsWanted::kTree ? plainMulLimit : k - 1;
+        const int L0 = k < 4 ? k : 4;
+        for (int i = plainMulLimit; i <= top && i < k && he == hipSuccess; ++i) {
+            const u32 *iv = scr + eval_inverse_offset(k) + (int64_t)(i - L0) * n;
+            he = steps_copy(&r.InvMl[i], iv, 1, nh, nh);
+            if (he == hipSuccess) he = steps_copy(&r.InvMr[i], iv + nh, 1, nh, nh);
+        }
+    }
+    if (eval && he == hipSuccess) {
+        he = steps_copy(&r.fL[1], dval, 1, nh, nh);
+        if (he == hipSuccess) he = steps_copy(&r.fR[1], dval + nh, 1, nh, nh);
+        for (int j = 2; keep_all && j <= k && he == hipSuccess; ++j) {
+            const u32 *rv = kept + (int64_t)(j - 1) * n;
+            he = steps_copy(&r.fL[j], rv, 1, nh, nh);
+            if (he == hipSuccess) he = steps_copy(&r.fR[j], rv + nh, 1, nh, nh);
+        }
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(0);
+    if (he != hipSuccess) return fail(CUMODP_FFT_ERROR, "evaluation steps", he);
+    return CUMODP_SUCCESS;
+}
+
+PolyEvalSteps steps_entry(const char *fn, const sfixn *F, const sfixn *points, int k, sfixn p,
+                          StepsWanted what, bool keep_all) {
+    PolyEvalSteps r;
+    memset(&r, 0, sizeof(r));
+    cumodp_err e = steps_run(r, F, points, k, p, what, keep_all);
+    if (e != CUMODP_SUCCESS) {
+        steps_free(r);
+        report(fn, e);
+    }
+    return r;
+}
+
+}  // namespace
+
+struct PolyEvalSteps fastEvaluation(sfixn *F, sfixn *points, sfixn k, sfixn p, sfixn flag) {
+    return steps_entry("fastEvaluation", F, points, k, p, StepsWanted::kEval, flag == 1);
+}
+
+struct PolyEvalSteps onlySubproductTree(sfixn *, sfixn *points, sfixn k, sfixn p, sfixn) {
+    return steps_entry("onlySubproductTree", nullptr, points, k, p, StepsWanted::kTree, false);
+}
+
+struct PolyEvalSteps onlyTrees(sfixn *points, sfixn k, sfixn p, sfixn) {
+    return steps_entry("onlyTrees", nullptr, points, k, p, StepsWanted::kTrees, false);
+}
+
 void stockham_dev(sfixn *X, sfixn n, sfixn k, sfixn w, sfixn p) {
     (void)n;
     report("stockham_dev", ntt_dev_impl((u32 *)X, 1, k, (u32)w, (u32)p, false, 0));

@@ -300,22 +300,76 @@ __global__ __launch_bounds__(kEvalParts << kEvalFuse) void k_inv_bottom(
     }
 }
 
+// F mod p: the remainder tree reads this copy, so F may hold any 32-bit words
+__global__ void k_reduce_copy(const u32 *__restrict__ F, int64_t n, u32 p, u32 *__restrict__ out) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+         g += (int64_t)gridDim.x * blockDim.x)
+        out[g] = F[g] % p;
+}
+
+// R_i(j) = R_(i+1)(j/2) mod M_i(j) for the levels below L0 (H = 2^i <= 8),
+// one thread per node, by schoolbook division by the monic M_i(j); used only
+// when every remainder level is kept
+template <int H>
+__global__ void k_rem_small(const u32 *__restrict__ parent, const u32 *__restrict__ M,
+                            int64_t nodes, u32 p, u32 *__restrict__ out) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nodes;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        u32 r[2 * H];
+#pragma unroll
+        for (int t = 0; t < 2 * H; ++t) r[t] = parent[(j >> 1) * (2 * H) + t];
+        const u32 *m = M + j * (H + 1);
+#pragma unroll
+        for (int d = 2 * H - 1; d >= H; --d) {
+            const u64 c = r[d] ? p - r[d] : 0u;
+#pragma unroll
+            for (int t = 0; t < H; ++t)
+                r[d - H + t] = (u32)((r[d - H + t] + c * m[t]) % p);
+        }
+#pragma unroll
+        for (int t = 0; t < H; ++t) out[j * H + t] = r[t];
+    }
+}
+
 int64_t eval_scratch_words(int k) {
     const int64_t n = (int64_t)1 << k;
     const int L0 = k < kEvalBase ? k : kEvalBase;
     const int nrev = k - 1 - L0 > 0 ? k - 1 - L0 : 0;          // reversed-node levels
-    return tree_level_offset(k, k + 1) + (int64_t)(k - L0 + 1) * n + 5 * n + (int64_t)nrev * n;
+    return tree_level_offset(k, k + 1) + (int64_t)(k - L0 + 1) * n + 6 * n + (int64_t)nrev * n;
 }
 
-hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u32 *values,
-                                  u32 *scratch, const Family *fam, hipStream_t s) {
+int64_t eval_inverse_offset(int k) { return tree_level_offset(k, k + 1); }
+
+// Level 0 in the reference's half-tree layout (leavesSubproductTree,
+// src/fastPolyEvaluation.cu:511-518): node j of the left half is
+// (-x_j, 1) and of the right half (-x_(nh+j), 1), -x the 32-bit negation
+__global__ void k_ref_leaves(const u32 *__restrict__ pts, int64_t nh, u32 *__restrict__ left,
+                             u32 *__restrict__ right) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nh;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        left[2 * j] = 0u - pts[j];
+        left[2 * j + 1] = 1u;
+        right[2 * j] = 0u - pts[nh + j];
+        right[2 * j + 1] = 1u;
+    }
+}
+
+hipError_t launch_ref_leaves(const u32 *pts, int64_t nh, u32 *left, u32 *right, hipStream_t s) {
+    hipLaunchKernelGGL(k_ref_leaves, dim3(eblocks(nh)), dim3(256), 0, s, pts, nh, left, right);
+    return hipGetLastError();
+}
+
+hipError_t launch_fast_evaluation(const u32 *F_in, const u32 *pts, int k, u32 p, u32 *values,
+                                  u32 *scratch, const Family *fam, hipStream_t s,
+                                  const EvalKeep *keep) {
     const int64_t n = (int64_t)1 << k;
     const int L0 = k < kEvalBase ? k : kEvalBase;
     u32 *tree = scratch;
     u32 *inv = tree + tree_level_offset(k, k + 1);           // levels L0 .. k, n words each
     u32 *R0 = inv + (int64_t)(k - L0 + 1) * n, *R1 = R0 + n;
     u32 *T1 = R1 + n, *T2 = T1 + n, *T3 = T2 + n;
-    u32 *rev = T3 + n;                                       // levels L0+1 .. k-1, n words each
+    u32 *Fr = T3 + n;                                        // F mod p
+    u32 *rev = Fr + n;                                       // levels L0+1 .. k-1, n words each
     auto inv_at = [&](int i) { return inv + (int64_t)(i - L0) * n; };
     auto node = [&](int i) { return tree + tree_level_offset(k, i); };
 
@@ -387,13 +441,19 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         e = launch_mul_strided(i + 1, I1, 2 * h, h, T3 + h, 2 * h, h, I1 + h, 2 * h, h, N, fam, s);
     }
     if (e != hipSuccess) return e;
-    // ---- remainder tree: R_k = F (read in place), then levels k-1 .. L0; the
+    if (keep && keep->trees_only) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_copy, dim3(eblocks(n)), dim3(256), 0, s, F_in, n, p, Fr);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const u32 *F = Fr;
+    u32 *kept = keep ? keep->rem : nullptr;
+    // ---- remainder tree: R_k = F mod p, then levels k-1 .. L0; the
     // reversed top halves of a level's dividends come from the previous
     // level's k_div_finish (from F at the first level)
     // levels below `stop` (from level pin = min(k, kEvalFuse) down) run in
     // k_rem_descend, one workgroup per node of level pin
     const int pin = k < kEvalFuse ? k : kEvalFuse;
-    const bool fuse = pin > L0;
+    const bool fuse = pin > L0 && !kept;
     const int stop = fuse ? pin : L0;
     const u32 *A = F;
     u32 *Rn = R0;
@@ -417,6 +477,9 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
         hipLaunchKernelGGL(k_div_finish, dim3(eblocks((int64_t)N * h)), dim3(256), 0, s, A, T2, h,
                            (int64_t)N, p, Rn, i > stop ? T1 : nullptr);
         e = hipGetLastError();
+        if (e == hipSuccess && kept)
+            e = hipMemcpyAsync(kept + (int64_t)i * n, Rn, sizeof(u32) * (size_t)n,
+                               hipMemcpyDeviceToDevice, s);
         A = Rn;
         Rn = Rn == R0 ? R1 : R0;
     }
@@ -430,7 +493,22 @@ hipError_t launch_fast_evaluation(const u32 *F, const u32 *pts, int k, u32 p, u3
     }
     hipLaunchKernelGGL(k_horner, dim3(eblocks(n)), dim3(256), 0, s, A, 1 << L0, pts, n,
                        plain_mod(p), values);
-    return hipGetLastError();
+    e = hipGetLastError();
+    // kept remainders below L0 (level L0 is in kept, or F itself when k = L0)
+    for (int i = L0 - 1; kept && i >= 0 && e == hipSuccess; --i) {
+        const u32 *par = i + 1 == k ? F : kept + (int64_t)(i + 1) * n;
+        const int64_t N = n >> i;
+        const u32 *Mi = node(i);
+        u32 *out = kept + (int64_t)i * n;
+        switch (i) {
+        case 3: hipLaunchKernelGGL(k_rem_small<8>, dim3(eblocks(N)), dim3(256), 0, s, par, Mi, N, p, out); break;
+        case 2: hipLaunchKernelGGL(k_rem_small<4>, dim3(eblocks(N)), dim3(256), 0, s, par, Mi, N, p, out); break;
+        case 1: hipLaunchKernelGGL(k_rem_small<2>, dim3(eblocks(N)), dim3(256), 0, s, par, Mi, N, p, out); break;
+        default: hipLaunchKernelGGL(k_rem_small<1>, dim3(eblocks(N)), dim3(256), 0, s, par, Mi, N, p, out); break;
+        }
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 }  // namespace cm

@@ -138,25 +138,28 @@ hipError_t launch_plain_list(const MulDesc *descs, int64_t count, const u32 *F, 
                              u32 *H, u32 p, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     if (count > INT32_MAX) return hipErrorInvalidValue;
-    std::vector<PlainChunk> chunks;
-    for (int64_t i = 0; i < count; ++i)
-        for (int64_t c0 = 0; c0 < descs[i].lh; c0 += kPlainChunk)
-            chunks.push_back(PlainChunk{(int32_t)i, (int32_t)c0});
-    if (chunks.size() > (size_t)INT32_MAX) return hipErrorInvalidValue;
+    int64_t nchunks = 0;
+    for (int64_t i = 0; i < count; ++i) nchunks += (descs[i].lh + kPlainChunk - 1) / kPlainChunk;
+    if (nchunks > INT32_MAX) return hipErrorInvalidValue;
+    // descriptors then chunks, written straight into the pinned staging buffer
     const size_t desc_bytes = sizeof(MulDesc) * (size_t)count;
-    const size_t chunk_bytes = sizeof(PlainChunk) * chunks.size();
-    std::vector<char> blob(desc_bytes + chunk_bytes);
-    memcpy(blob.data(), descs, desc_bytes);
-    memcpy(blob.data() + desc_bytes, chunks.data(), chunk_bytes);
-    char *dblob = (char *)workspace(17, blob.size(), s);
+    const size_t bytes = desc_bytes + sizeof(PlainChunk) * (size_t)nchunks;
+    char *dblob = (char *)workspace(17, bytes, s);
     if (!dblob) return hipErrorOutOfMemory;
-    hipError_t e = stage_upload(blob.data(), blob.size(), dblob, s);
+    hipError_t e = stage_upload_fill(bytes, dblob, s, [&](char *blob) {
+        memcpy(blob, descs, desc_bytes);
+        PlainChunk *ch = (PlainChunk *)(blob + desc_bytes);
+        for (int64_t i = 0; i < count; ++i)
+            for (int32_t c0 = 0; c0 < descs[i].lh; c0 += kPlainChunk)
+                *ch++ = PlainChunk{(int32_t)i, c0};
+    });
     if (e != hipSuccess) return e;
+    const size_t chunk_count = (size_t)nchunks;
     const MulDesc *d_descs = (const MulDesc *)dblob;
     const PlainChunk *d_chunks = (const PlainChunk *)(dblob + desc_bytes);
     const PlainMod m = plain_mod(p);
     ProfScope ps(kProfPlainMul, s);
-    const dim3 g((unsigned)chunks.size());
+    const dim3 g((unsigned)chunk_count);
     hipLaunchKernelGGL(k_plain_mul<false>, g, dim3(kPlainLanes), 0, s, F, G, H, d_descs,
                        d_chunks, MulDesc{}, 0, m);
     return hipGetLastError();

@@ -57,7 +57,7 @@ Family *make_family(u32 p, u32 wN, int K) {
     f->np = cm_host_npinv(p);
     f->K = K;
     f->wN = wN;
-    f->LT = K < kMaxRowLog ? K : kMaxRowLog;
+    f->LT = K < kMaxLevelLog ? K : kMaxLevelLog;
     // level tables: level M = 2^j (j = 1 .. LT) at offset 2^(j-1) - 1
     std::vector<u32> lv, lvi;
     for (int j = 1; j <= f->LT; ++j) {
@@ -276,6 +276,11 @@ std::map<hipStream_t, std::vector<void *>> g_captured;
 }  // namespace
 
 hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s) {
+    return stage_upload_fill(bytes, dst, s, [&](char *h) { memcpy(h, src, bytes); });
+}
+
+hipError_t stage_upload_fill(size_t bytes, void *dst, hipStream_t s,
+                             const std::function<void(char *)> &fill) {
     std::lock_guard<std::mutex> lk(g_ws_mutex);
     hipError_t e;
     if (stream_capturing(s)) {
@@ -283,7 +288,7 @@ hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s)
         void *h = nullptr;
         e = hipHostMalloc(&h, bytes ? bytes : 1);
         if (e != hipSuccess) return e;
-        memcpy(h, src, bytes);
+        fill((char *)h);
         g_captured[s].push_back(h);
         return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s);
     }
@@ -318,7 +323,7 @@ hipError_t stage_upload(const void *src, size_t bytes, void *dst, hipStream_t s)
         if (e != hipSuccess) return e;
         st.cap = bytes;
     }
-    memcpy(st.host, src, bytes);
+    fill((char *)st.host);
     e = hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     return hipEventRecord(st.done, s);

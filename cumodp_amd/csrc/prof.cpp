@@ -20,7 +20,7 @@ namespace {
 const char *kNames[kProfCount] = {
     "k_col_fwd", "k_row_mul", "k_col_inv", "k_mul_lds", "k_mul_mixed",
     "k_ntt_lds", "k_row_out", "k_gather_scatter", "k_twoconv_split", "k_twoconv_crt",
-    "k_col_ntt", "k_elementwise", "k_plain_mul",
+    "k_col_ntt", "k_elementwise", "k_plain_mul", "k_list_persistent",
 };
 
 struct Rec {

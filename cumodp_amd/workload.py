@@ -26,7 +26,7 @@ import numpy as np
 
 __all__ = ["SplitMix64", "uniform_mod", "signed_uniform", "log_uniform_int",
            "P1", "P2", "workload1", "workload2", "workload3", "workload4",
-           "workload4_tree", "workload5"]
+           "workload4_tree", "workload5", "exp_gaps"]
 
 _GOLD = np.uint64(0x9E3779B97F4A7C15)
 P1 = 469762049       # 7 * 2^26 + 1
@@ -132,6 +132,12 @@ def workload4(pairs=4096, lo=32, hi=16384):
 def workload4_tree(logn=16):
     rng = SplitMix64(40)
     return {"p": P1, "points": uniform_mod(rng, 1 << logn, P1)}
+
+
+def exp_gaps(seed, n):
+    """n exponential inter-arrival gaps of mean 1 from SplitMix64(seed)."""
+    rng = SplitMix64(seed)
+    return -np.log1p(-((rng.next(n) >> np.uint64(11)).astype(np.float64) / (1 << 53)))
 
 
 def workload5(requests=20000, lo=64, hi=65536):

@@ -214,6 +214,12 @@ CUMODP_API int cumodp_tune_get_list_tile(void);
  * 256; 0 sends every product to the NTT; at most 1024) */
 CUMODP_API cumodp_err cumodp_tune_set_list_plain(int cutoff);
 CUMODP_API int cumodp_tune_get_list_plain(void);
+/* cumodp_list_fftmul_dev runs every product of up to 2^15 points, schoolbook
+ * and NTT, in one persistent launch (1, default; workgroups take tasks
+ * largest first from a shared counter), or one launch per size class (0).
+ * Same results.  CUMODP_LIST_PERSISTENT=0 in the environment sets 0. */
+CUMODP_API cumodp_err cumodp_tune_set_list_persistent(int on);
+CUMODP_API int cumodp_tune_get_list_persistent(void);
 /* cumodp_two_convolution_list_dev groups products by plan: 0 (default) one
  * group per exact plan; 1 one group per transform length, with the plan of
  * the widest coefficients of that length (same results, fewer launches, more
