@@ -292,25 +292,29 @@ def bench_list_multiply(cm, cpu, reps=5):
     H1, H2 = dH1.download(), dH2.download()
     # the same batched call captured once into a graph and replayed
     L = cm.lib()
-    st = L.cumodp_stream_create()
-    dH3 = cm.DeviceArray(nh)
-    g = cm.Graph.capture(st, lambda: cm.list_fftmul_dev(items, dF, dG, dH3, p, st))
-    tg = timed(lambda: g.launch(st), reps, lambda: L.cumodp_stream_synchronize(st))
-    H3 = dH3.download()
-    g.destroy()
-    L.cumodp_stream_destroy(st)
-    for d in (dF, dG, dH1, dH2, dH3):
+    tg, H3 = None, None
+    if GRAPHS:
+        st = L.cumodp_stream_create()
+        dH3 = cm.DeviceArray(nh)
+        g = cm.Graph.capture(st, lambda: cm.list_fftmul_dev(items, dF, dG, dH3, p, st))
+        tg = timed(lambda: g.launch(st), reps, lambda: L.cumodp_stream_synchronize(st))
+        H3 = dH3.download()
+        g.destroy()
+        L.cumodp_stream_destroy(st)
+        dH3.free()
+    for d in (dF, dG, dH1, dH2):
         d.free()
     out = {"workload": "4096 pairs, degrees log-uniform [32, 16384], mod 469762049",
            "output_coeffs": nh,
            "batched": {"ms": tb * 1e3, "coeffs_per_s": nh / tb},
            "per_call": {"ms": tc * 1e3, "coeffs_per_s": nh / tc},
-           "graph_replay": {"ms": tg * 1e3, "coeffs_per_s": nh / tg},
+           "graph_replay": tg and {"ms": tg * 1e3, "coeffs_per_s": nh / tg},
            "batched_over_per_call": tc / tb,
            "valu_issue_batched": valu_issue("list_multiply", tb),
            "check_batched_eq_per_call": bool(np.array_equal(H1, H2)),
-           "check_graph_eq_batched": bool(np.array_equal(H1, H3)),
-           "tuning": {"list_tile_words": L.cumodp_tune_get_list_tile(),
+           "check_graph_eq_batched": None if H3 is None else bool(np.array_equal(H1, H3)),
+           "tuning": {"list_persistent": L.cumodp_tune_get_list_persistent(),
+                      "list_tile_words": L.cumodp_tune_get_list_tile(),
                       "list_plain_cutoff": L.cumodp_tune_get_list_plain()}}
     if cpu:
         # the reference's own CPU multiply (modpn EX_Mont_Mul, built from its

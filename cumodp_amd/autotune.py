@@ -132,7 +132,7 @@ def tune_list_persistent(reps=5, log=None, pairs=4096):
     return b
 
 
-def tune_list_plain(reps=5, log=None, pairs=4096, cutoffs=(0, 64, 128, 192, 256, 384)):
+def tune_list_plain(reps=5, log=None, pairs=4096, cutoffs=(0, 32, 64, 128, 192, 256, 384)):
     """Time the uneven list multiply (the 4096-pair workload) for each plain
     cut-off; apply and return the best."""
     L = lib()
@@ -224,7 +224,7 @@ def save(choices, path):
         json.dump({"split": {str(k): v for k, v in choices["split"].items()},
                    "list_tile_words": choices["list_tile_words"],
                    "col_tile_words": choices.get("col_tile_words", 16384),
-                   "list_plain_cutoff": choices.get("list_plain_cutoff", 256),
+                   "list_plain_cutoff": choices.get("list_plain_cutoff", 32),
                    "list_persistent": choices.get("list_persistent", 1)}, f, indent=1)
 
 

@@ -401,8 +401,11 @@ int list_tile_words() { return kMixWordsOf[g_mix_variant]; }
 // Products whose shorter operand has at most this many coefficients run on
 // the schoolbook kernel (plain.hip) in list multiplies: there the la * lb
 // products cost less than three transforms of 2^ceil(log2(la + lb - 1))
-// points.  0 sends everything to the NTT.  The tuner sets it.
-int g_plain_cutoff = 256;
+// points.  0 sends everything to the NTT.  The tuner sets it.  Measured on
+// the 4096 uneven pairs: 32 in the one-launch form (its transform tasks run
+// at about two thirds of the VALU peak, its schoolbook tasks well below:
+// 0.29 ms per call against 0.34 ms at 256), 256 in the per-class form.
+int g_plain_cutoff = 32;
 
 bool set_list_plain_cutoff(int c) {
     if (c < 0 || c > 1024) return false;
@@ -411,6 +414,11 @@ bool set_list_plain_cutoff(int c) {
 }
 
 int list_plain_cutoff() { return g_plain_cutoff; }
+
+// The strided multiplies of tree and evaluation levels (equal shapes) take
+// the schoolbook kernel up to this shorter-operand length: the measured
+// crossover of equal-length pairs (plain faster up to 256 coefficients).
+int strided_plain_cutoff() { return 256; }
 
 // smallest class of 2^14-point products that runs in LDS (one workgroup per
 // CU at a time, so it needs a few per CU to pay)

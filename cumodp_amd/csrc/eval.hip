@@ -433,7 +433,7 @@ hipError_t launch_fast_evaluation(const u32 *F_in, const u32 *pts, int k, u32 p,
         // terms land below x^(h-1)); short products take the schoolbook
         // kernel on 4h points instead (same words h .. 2h-1)
         const u32 *fneg = rev + (int64_t)(i - L0) * n;
-        const int ln2 = h <= list_plain_cutoff() ? i + 2 : i + 1;
+        const int ln2 = h <= strided_plain_cutoff() ? i + 2 : i + 1;
         e = launch_mul_strided(ln2, fneg, 2 * h, 2 * h, I1, 2 * h, h, T3, 2 * h, 2 * h, N, fam, s);
         if (e != hipSuccess) break;
         // high half of I_(i+1)(j) = g0 (-E) mod x^h (reads the low halves of

@@ -941,7 +941,7 @@ hipError_t launch_mul_strided(int logn, const u32 *F, int64_t fs, int lf, const 
     if (count <= 0) return hipSuccess;
     if (logn < 4) logn = 4;
     // short operands, no wrap-around: the schoolbook kernel (same words)
-    if (std::min(lf, lg) <= list_plain_cutoff() && (int64_t)lf + lg - 1 <= ((int64_t)1 << logn) &&
+    if (std::min(lf, lg) <= strided_plain_cutoff() && (int64_t)lf + lg - 1 <= ((int64_t)1 << logn) &&
         lh <= lf + lg - 1)
         return launch_plain_strided(F, fs, lf, G, gs, lg, H, hs, lh, count, fam->p, s);
     if (logn <= kMaxLdsLog) {

@@ -211,7 +211,8 @@ CUMODP_API cumodp_err cumodp_tune_set_list_tile(int words);
 CUMODP_API int cumodp_tune_get_list_tile(void);
 /* cumodp_list_fftmul_dev sends products whose shorter operand has at most
  * `cutoff` coefficients to the schoolbook kernel (same results; default
- * 256; 0 sends every product to the NTT; at most 1024) */
+ * 32, the measured best of the one-launch form; 256 suits the per-class
+ * form; 0 sends every product to the NTT; at most 1024) */
 CUMODP_API cumodp_err cumodp_tune_set_list_plain(int cutoff);
 CUMODP_API int cumodp_tune_get_list_plain(void);
 /* cumodp_list_fftmul_dev runs every product of up to 2^15 points, schoolbook
