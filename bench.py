@@ -569,7 +569,7 @@ def _eval_check(a, b, cw, x, q):
     return ev(_words_mod(cw, q)) == ev(am) * ev(bm) % q
 
 
-def bench_serving(cm, cpu, nreq=20000, max_batch_ms=2.0):
+def bench_serving(cm, cpu, nreq=20000, max_batch_ms=1.5):
     """Exact integer products served as they arrive: 20000 requests, degrees
     log-uniform in [64, 65536], coefficients of 16/32/48/64 bits (uniform
     choice), signed uniform.  First all requests arrive at once (capacity);
@@ -754,7 +754,7 @@ def main():
                          "list tile) before the timed runs")
     ap.add_argument("--serving-requests", type=int, default=20000,
                     help="requests of the serving line (0 skips it)")
-    ap.add_argument("--serving-max-batch-ms", type=float, default=2.0,
+    ap.add_argument("--serving-max-batch-ms", type=float, default=1.5,
                     help="service-time cap of a serving batch under Poisson arrivals")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (reference for "
