@@ -36,9 +36,7 @@ cumodp_err fail(cumodp_err code, const char *what, hipError_t e = hipSuccess) {
 }
 
 int ceil_log2(int64_t s) {
-    int e = 0;
-    while (((int64_t)1 << e) < s) ++e;
-    return e;
+    return s <= 1 ? 0 : 64 - __builtin_clzll((unsigned long long)(s - 1));
 }
 
 hipStream_t as_stream(void *s) { return (hipStream_t)s; }
@@ -873,7 +871,9 @@ cumodp_err cumodp_list_fftmul_dev(const cumodp_mul_item *items, int64_t count, c
                                   const sfixn *G, sfixn *H, sfixn p, void *stream) {
     if (count < 0) return fail(CUMODP_ASSUMPTION_ERROR, "negative count");
     if (count == 0) return CUMODP_SUCCESS;
-    std::vector<MulDesc> d((size_t)count);
+    // the descriptor array is reused from call to call (this thread's)
+    thread_local std::vector<MulDesc> d;
+    d.resize((size_t)count);
     int emax = 0;
     for (int64_t i = 0; i < count; ++i) {
         const cumodp_mul_item &it = items[i];

@@ -378,10 +378,15 @@ __global__ __launch_bounds__(kPT) void k_list_persistent(
 
 namespace {
 
-// pointwise constants of the 2^e-point products, e = 0 .. 15
+// pointwise constants of the 2^e-point products, e = 0 .. 15 (those of the
+// last modulus are kept; callers hold the API lock)
 Scales scale_table(const Family *fam) {
-    Scales t;
-    for (int e = 0; e < 16; ++e) t.v[e] = mul_scale(e, fam->p);
+    static u32 last_p = 0;
+    static Scales t;
+    if (last_p != fam->p) {
+        for (int e = 0; e < 16; ++e) t.v[e] = mul_scale(e, fam->p);
+        last_p = fam->p;
+    }
     return t;
 }
 
