@@ -207,26 +207,41 @@ class ZmulServer:
         ey = max(4, (int(ny) - 1).bit_length())
         return self._scratch_bytes(bits, ny) * ey / 4.0
 
+    def _costs(self, Q):
+        """Prefix sums over the requests of Q of their footprint bytes
+        (inputs, outputs, estimated transform scratch) and work units, kept
+        for the last Q seen: admission is then two binary searches, with no
+        per-request Python work while the GPU waits."""
+        if getattr(self, "_cost_q", None) is not Q:
+            # y-transform exponent per request: max(4, bit_length(nc - 1))
+            ey = np.maximum(4, np.frexp((Q.nc - 1).astype(np.float64))[1]).astype(np.int64)
+            scratch = np.zeros(len(Q.nc), np.float64)
+            for b, e in set(zip(Q.bits.tolist(), ey.tolist())):
+                sel = (Q.bits == b) & (ey == e)
+                scratch[sel] = self._scratch_bytes(b, (1 << (e - 1)) + 1)
+            nbytes = 8.0 * (Q.na + Q.nb + Q.nc * Q.wc) + scratch
+            units = scratch * ey / 4.0
+            cb = np.zeros(len(Q.nc) + 1)
+            cu = np.zeros(len(Q.nc) + 1)
+            np.cumsum(nbytes, out=cb[1:])
+            np.cumsum(units, out=cu[1:])
+            self._cost_q, self._cost = Q, (cb, cu)
+        return self._cost
+
     def _fit(self, Q, i0, i1):
         """Largest j in (i0, i1] such that requests [i0, j) fit one slot: the
-        footprint cap and, once the rate is known, the service-time cap."""
-        total = 0
-        work = 0.0
-        cap_units = None
+        footprint cap and, once the rate is known, the service-time cap (the
+        first request is always admitted)."""
+        cb, cu = self._costs(Q)
+        j = min(i1, int(np.searchsorted(cb, cb[i0] + self.slot_bytes, side="right")) - 1)
         if self.max_batch_s is not None and self.sec_per_unit:
             cap_units = self.max_batch_s / self.sec_per_unit
-        j = i0
-        while j < i1:
-            b = 8 * (int(Q.na[j] + Q.nb[j]) + int(Q.nc[j] * Q.wc[j])) + \
-                self._scratch_bytes(Q.bits[j], Q.nc[j])
-            u = self._units(Q.bits[j], Q.nc[j])
-            if j > i0 and (total + b > self.slot_bytes or
-                           (cap_units is not None and work + u > cap_units)):
-                break
-            total += b
-            work += u
-            j += 1
-        return j
+            j = min(j, int(np.searchsorted(cu, cu[i0] + cap_units, side="right")) - 1)
+        return max(j, i0 + 1)
+
+    def _batch_units(self, Q, i0, i1):
+        cu = self._costs(Q)[1]
+        return float(cu[i1] - cu[i0])
 
     def _learn(self, units, seconds):
         """Fold one retired batch's measured service time into the rate."""
@@ -254,7 +269,7 @@ class ZmulServer:
         _check("cumodp_memcpy_d2h_async", L.cumodp_memcpy_d2h_async(
             C.c_void_p(Q.OUT.ptr + 8 * out0), C.c_void_p(slot.dout), 8 * (out1 - out0), st))
         slot.range = (i0, i1)
-        slot.units = sum(self._units(Q.bits[r], Q.nc[r]) for r in range(i0, i1))
+        slot.units = self._batch_units(Q, i0, i1)
 
     def run(self, Q, arrivals):
         """Serve requests 0..Q.n-1 arriving at `arrivals` (seconds from the

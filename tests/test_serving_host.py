@@ -76,3 +76,28 @@ def test_latency_summary():
     s = serving.latency_summary(done, arr)
     assert abs(s["p50_ms"] - 2.5) < 1e-9
     assert abs(s["max_ms"] - 4.0) < 1e-9
+
+
+def test_admission_matches_request_by_request_counting():
+    """The prefix-sum admission admits exactly the requests that adding them
+    one at a time under both caps would admit."""
+    rng = np.random.default_rng(7)
+    n = 300
+    Q = _fake_requests(rng.integers(1, 5000, n), rng.integers(1, 5000, n),
+                       rng.choice([16, 32, 48, 64], n))
+    for cap_bytes, cap_s in ((1 << 22, None), (1 << 24, 2e-4), (1 << 40, 5e-5), (1 << 21, 1e-3)):
+        s = _server(cap_bytes)
+        s.max_batch_s = cap_s
+        s.sec_per_unit = 1e-12
+        for i0 in range(0, n - 1, 17):
+            i1 = min(n, i0 + int(rng.integers(1, 120)))
+            total, work, j = 0, 0.0, i0
+            while j < i1:
+                b = 8 * (int(Q.na[j] + Q.nb[j]) + int(Q.nc[j] * Q.wc[j])) + \
+                    s._scratch_bytes(Q.bits[j], Q.nc[j])
+                u = s._units(Q.bits[j], Q.nc[j])
+                if j > i0 and (total + b > cap_bytes or
+                               (cap_s is not None and work + u > cap_s / s.sec_per_unit)):
+                    break
+                total, work, j = total + b, work + u, j + 1
+            assert s._fit(Q, i0, i1) == j, (cap_bytes, cap_s, i0, i1)
