@@ -569,13 +569,14 @@ def _eval_check(a, b, cw, x, q):
     return ev(_words_mod(cw, q)) == ev(am) * ev(bm) % q
 
 
-def bench_serving(cm, cpu, nreq=20000, max_batch_ms=1.5):
+def bench_serving(cm, cpu, nreq=20000, max_batch_ms=1.5, age_factor=0.5):
     """Exact integer products served as they arrive: 20000 requests, degrees
     log-uniform in [64, 65536], coefficients of 16/32/48/64 bits (uniform
     choice), signed uniform.  First all requests arrive at once (capacity);
     then three runs of Poisson arrivals at 80% of that capacity (exponential
     gaps from the workload stream and from two more seeds), with batches
-    capped at max_batch_ms of estimated GPU work.  Latency = time from
+    capped at max_batch_ms of estimated GPU work, or at age_factor times the
+    oldest waiting request's age when that is longer.  Latency = time from
     arrival to the product being back in host memory."""
     from cumodp_amd import serving
     t0 = time.perf_counter()
@@ -583,7 +584,7 @@ def bench_serving(cm, cpu, nreq=20000, max_batch_ms=1.5):
     Q = serving.make_requests(w)
     log(f"bench: serving: {Q.n} requests generated in {time.perf_counter() - t0:.1f} s, "
         f"{Q.total_coeffs} output coefficients")
-    srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30)
+    srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30, age_factor=age_factor)
     runs = []
     try:
         srv.run(Q, np.zeros(Q.n))   # warm-up pass (plans, twiddles, buffers)
@@ -618,7 +619,8 @@ def bench_serving(cm, cpu, nreq=20000, max_batch_ms=1.5):
            "capacity": {"s": cap_s, "coeffs_per_s": Q.total_coeffs / cap_s,
                         "requests_per_s": cap_rps, "valu_issue": valu_issue("serving", cap_s),
                         **st0},
-           "poisson": {"rate_requests_per_s": lam, "max_batch_ms": max_batch_ms, **lat,
+           "poisson": {"rate_requests_per_s": lam, "max_batch_ms": max_batch_ms,
+                       "age_factor": age_factor, **lat,
                        "runs": runs},
            "check_evaluation_sample": bool(ok)}
     if cpu:
@@ -715,7 +717,7 @@ def counter_calls(cm, line, ncalls):
     elif line == "serving":
         from cumodp_amd import serving
         Q = serving.make_requests(cm.workload5())
-        srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30)
+        srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30, age_factor=age_factor)
         run = lambda: srv.run(Q, np.zeros(Q.n))
     else:
         raise SystemExit(f"no counter form for line {line}")

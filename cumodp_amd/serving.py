@@ -26,7 +26,10 @@ Design:
     as an exponentially weighted average. Without this cap a burst makes
     one batch of hundreds of requests, every request in it waits for all
     of them, and the arrivals during that long batch make the next one as
-    long.
+    long.  With `age_factor` a, the cap grows to a times the time the oldest
+    waiting request has waited when that is longer: a backlog left by a
+    burst then drains in fewer, larger (more efficient) batches instead of
+    at the rate of capped batches, which barely exceeds the arrival rate.
   * Twiddle tables stay resident in the library across batches.
 
 The reference has no serving layer. This drives the integer multiply that
@@ -172,12 +175,15 @@ class _Slot:
 class ZmulServer:
     """Continuous batching over `slots` streams; a batch holds at most
     `slot_bytes` of inputs, outputs and estimated transform scratch, and at
-    most `max_batch_s` seconds of estimated GPU work (None: no time cap)."""
+    most max(`max_batch_s`, `age_factor` x the oldest waiting request's age)
+    seconds of estimated GPU work (max_batch_s None: no time cap)."""
 
-    def __init__(self, slots=2, slot_bytes=1 << 30, max_batch_s=None):
+    def __init__(self, slots=2, slot_bytes=1 << 30, max_batch_s=None, age_factor=0.0):
         self.slots = [_Slot() for _ in range(slots)]
         self.slot_bytes = int(slot_bytes)
         self.max_batch_s = max_batch_s
+        self.age_factor = age_factor
+        self._age = 0.0   # how long the oldest waiting request has waited
         self.sec_per_unit = None     # learnt from served batches
         self._plans = {}
 
@@ -235,7 +241,8 @@ class ZmulServer:
         cb, cu = self._costs(Q)
         j = min(i1, int(np.searchsorted(cb, cb[i0] + self.slot_bytes, side="right")) - 1)
         if self.max_batch_s is not None and self.sec_per_unit:
-            cap_units = self.max_batch_s / self.sec_per_unit
+            cap_s = max(self.max_batch_s, self.age_factor * self._age)
+            cap_units = cap_s / self.sec_per_unit
             j = min(j, int(np.searchsorted(cu, cu[i0] + cap_units, side="right")) - 1)
         return max(j, i0 + 1)
 
@@ -308,6 +315,7 @@ class ZmulServer:
                 free.append(s)
             arrived = int(np.searchsorted(arrivals, now, side="right"))
             if head < arrived and free:
+                self._age = now - arrivals[head]
                 j = self._fit(Q, head, arrived)
                 s = free.pop(0)
                 s.t_launch = time.perf_counter() - t0

@@ -30,6 +30,8 @@ def _server(cap):
     s._plans = {}
     s.max_batch_s = None
     s.sec_per_unit = None
+    s.age_factor = 0.0
+    s._age = 0.0
     # scratch estimate without the library: 2 x 8 x Ly words per prime, 2 primes
     s._scratch_bytes = lambda bits, ny: 4 * 2 * 8 * (1 << max(4, (int(ny) - 1).bit_length())) * 2
     return s
