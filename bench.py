@@ -717,7 +717,7 @@ def counter_calls(cm, line, ncalls):
     elif line == "serving":
         from cumodp_amd import serving
         Q = serving.make_requests(cm.workload5())
-        srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30, age_factor=age_factor)
+        srv = serving.ZmulServer(slots=2, slot_bytes=1 << 30)   # the capacity run: no time cap
         run = lambda: srv.run(Q, np.zeros(Q.n))
     else:
         raise SystemExit(f"no counter form for line {line}")
