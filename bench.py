@@ -82,7 +82,7 @@ def cpu_baseline(F, G, p, reps=2):
 
 RAMP_S = 0.25
 VALU_PEAK = 1024 * 2.32e9 / 4    # wave-instructions/s: 1024 SIMDs, 4 cycles each, 2.32 GHz
-CPU_THREADS = 16                 # the GPU box's host-core share for the OpenMP CPU legs
+CPU_THREADS = 16                 # threads of the OpenMP CPU legs (the host-core share of one GPU)
 
 
 def source_hash():
