@@ -103,3 +103,20 @@ def test_admission_matches_request_by_request_counting():
                     break
                 total, work, j = total + b, work + u, j + 1
             assert s._fit(Q, i0, i1) == j, (cap_bytes, cap_s, i0, i1)
+
+
+def test_service_time_cap_grows_with_queue_age():
+    """With age_factor a, the cap is max(max_batch_s, a x the oldest waiting
+    request's age): a backlog is served in larger batches."""
+    Q = _fake_requests([100] * 10, [100] * 10, [64] * 10)
+    s = _server(1 << 40)
+    s.max_batch_s = 1e-3
+    u = s._units(64, 199)
+    s._learn(u, 1e-3)                       # 1 ms per request
+    s.age_factor = 0.5
+    s._age = 0.0
+    assert s._fit(Q, 0, 10) == 1            # young queue: the fixed cap
+    s._age = 6.5e-3                         # oldest request waited 6.5 ms
+    assert s._fit(Q, 0, 10) == 3            # cap 3.25 ms: three requests
+    s.age_factor = 0.0
+    assert s._fit(Q, 0, 10) == 1
