@@ -919,6 +919,22 @@ def main():
                    "col_tile_words": L.cumodp_tune_get_col_tile()},
         "check": {"evaluation_identity": ok_eval},
     }
+    # the reference's boundary takes host buffers: the same multiply through
+    # cumodp_fftmul_uni on host arrays (pageable memory), copies included;
+    # reported beside `value`, never as it
+    Hh = cm.poly_mul(F, G, p)
+    th = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        Hh = cm.poly_mul(F, G, p)
+        th.append(time.perf_counter() - t0)
+    t_host = float(np.median(th))
+    out["host_api_with_copies"] = {
+        "ms": t_host * 1e3, "coeffs_per_s": nout / t_host,
+        "call": "cumodp_fftmul_uni on pageable host arrays: both operands in, the product out "
+                "(2 x 32 MiB up, 64 MiB down), median of 5",
+        "check_equals_device_result": bool(np.array_equal(Hh, H))}
+    log(f"bench: host entry point with copies {t_host * 1e3:.2f} ms")
     if not args.no_cpu_baseline:
         log("bench: cpu baseline (oracle serial NTT multiply, 1 thread)")
         Hc, tc = cpu_baseline(F, G, p)
