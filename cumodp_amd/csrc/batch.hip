@@ -647,10 +647,13 @@ static int device_cus() {
     return cached[dev];
 }
 
-// floor(log2(x)) of a cost, 0 .. 63
+// floor(log2(x)) of a cost, 0 .. 63: the exponent field of the double
+// (a library ilogb call per task cost a few microseconds per list call)
 static int cost_bucket(double x) {
     if (!(x >= 1.0)) return 0;
-    const int b = std::ilogb(x);
+    uint64_t u;
+    memcpy(&u, &x, sizeof u);
+    const int b = (int)((u >> 52) & 0x7ff) - 1023;
     return b > 63 ? 63 : b;
 }
 
