@@ -422,11 +422,40 @@ def bench_tree(cm, cpu, reps=5):
     root = tree[L.cumodp_subproduct_tree_offset(k, k):]
     r = 123456789
     ok = eval_mod(root, r, p) == _prod_mod([(r - int(x)) % p for x in pts], p)
-    dP.free(), dT.free()
+    # the plain per-call form: the same tree from the same level 0, one
+    # cumodp_fftmul_uni_dev call per node product (2^16 - 1 calls), level by
+    # level on one stream; timed once after one untimed run
+    offs = [L.cumodp_subproduct_tree_offset(k, i) for i in range(k + 1)]
+    dT2 = cm.DeviceArray.from_host(np.concatenate([tree[:offs[1]],
+                                                   np.zeros(nw - offs[1], tree.dtype)]))
+    base = dT2.addr().value
+
+    def per_call():
+        for i in range(1, k + 1):
+            h = 1 << (i - 1)
+            src, dst = base + 4 * offs[i - 1], base + 4 * offs[i]
+            for j in range(1 << (k - i)):
+                f = src + 4 * 2 * j * (h + 1)
+                cm.fftmul_uni_dev(2 * h, dst + 4 * j * (2 * h + 1), h, f, h, f + 4 * (h + 1), p)
+
+    per_call()
+    cm.synchronize()
+    t0 = time.perf_counter()
+    per_call()
+    cm.synchronize()
+    tc = time.perf_counter() - t0
+    same_pc = bool(np.array_equal(dT2.download(), tree))
+    dP.free(), dT.free(), dT2.free()
+    coeffs = sum((1 << (k - i)) * ((1 << i) + 1) for i in range(1, k + 1))
     out = {"workload": "subproduct tree, 2^16 points uniform mod 469762049, levels 0..16",
            "ms": t * 1e3, "graph_replay_ms": tg and tg * 1e3, "tree_words": nw,
+           "coeffs_per_s": coeffs / t,
+           "per_call": {"ms": tc * 1e3, "coeffs_per_s": coeffs / tc,
+                        "calls": (1 << k) - 1},
+           "batched_over_per_call": tc / t,
            "valu_issue": valu_issue("subproduct_tree", t),
-           "check_root_evaluation": bool(ok), "check_graph_eq_eager": same_g}
+           "check_root_evaluation": bool(ok), "check_graph_eq_eager": same_g,
+           "check_per_call_eq_batched": same_pc}
     if cpu:
         import oracle
         oracle.lib()
