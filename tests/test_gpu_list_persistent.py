@@ -7,7 +7,8 @@ Every task kind of the persistent kernel is covered:
   longer and shorter than 2^14 on either side;
 * schoolbook products, including outputs longer than one 4096-word task and
   (cut-off 1024) shorter operands longer than one 512-term pass;
-* 2^16-point products after the persistent launch (batched four-step).
+* 2^16-point products after the persistent launch (batched four-step);
+* 60,000 small products in one call (tens of thousands of workgroups).
 Both primes below 2^30 (lazy butterflies) and above (canonical) run.
 """
 import numpy as np
@@ -112,3 +113,31 @@ def test_persistent_repeated_calls_and_graph(cm, orc):
     assert np.array_equal(first[it.h_off:it.h_off + it.df + it.dg + 1],
                           orc.poly_mul(F[it.f_off:it.f_off + it.df + 1],
                                        G[it.g_off:it.g_off + it.dg + 1], p))
+
+
+def test_persistent_many_small_products(cm, orc):
+    """60,000 products of degree below 300 (schoolbook and tile tasks of
+    every small size, tens of thousands of workgroups, a descriptor blob of
+    several MiB): the persistent launch equals the per-class launches, and
+    a spread sample equals the oracle."""
+    L = cm.lib()
+    p = 469762049
+    rng = np.random.default_rng(35)
+    n = 60000
+    df = rng.integers(0, 300, n)
+    dg = rng.integers(0, 300, n)
+    items, nf, ng, nh = cm.make_items(df, dg)
+    F = rng.integers(0, p, nf).astype(np.int32)
+    G = rng.integers(0, p, ng).astype(np.int32)
+    dF, dG = cm.DeviceArray.from_host(F), cm.DeviceArray.from_host(G)
+    try:
+        H1 = _run(cm, L, items, dF, dG, nh, p, 1)
+        H0 = _run(cm, L, items, dF, dG, nh, p, 0)
+    finally:
+        dF.free(), dG.free()
+    assert np.array_equal(H1, H0)
+    for i in np.linspace(0, n - 1, 40).astype(int):
+        it = items[int(i)]
+        f = F[it.f_off:it.f_off + it.df + 1]
+        g = G[it.g_off:it.g_off + it.dg + 1]
+        assert np.array_equal(H1[it.h_off:it.h_off + it.df + it.dg + 1], orc.poly_mul(f, g, p)), i
